@@ -1,0 +1,122 @@
+/* hvm_hip.h — C ABI of the MI355X HVM2 evaluator (libhvm_hip.so).
+ *
+ * The reference's host calls its GPU runtime through one entry,
+ *     extern "C" void hvm_cu(u32* book_buffer);        src/hvm.cu:2320
+ * declared on the Rust side at src/main.rs:17-20 and called at
+ * src/main.rs:93-96 with the bytes of Book::to_buffer (src/hvm.rs:957-1003).
+ * `hvm_hip` below is the drop-in for it.  The remaining entries expose the
+ * same evaluation without stdout parsing (status codes, counters, result
+ * text), a persistent context for repeated runs, and the front end that the
+ * reference keeps in Rust (src/ast.rs), which this build provides in C++.
+ *
+ * All pointers are plain host pointers; no torch or HIP types cross the ABI.
+ */
+#ifndef HVM_HIP_H
+#define HVM_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in entry                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces `hvm_cu` (src/hvm.cu:2320-2397) and `hvm_c` (src/hvm.c:1743-1791).
+ * Loads the book (src/hvm.cu:1898-1947 semantics; prints "failed to load
+ * book" to stderr and returns on failure), evaluates @main on the GPU with
+ * the run-cu IO driver's readback (src/run.cu:857-925) and prints, exactly
+ * as run-cu does:
+ *     Result: <tree>\n- ITRS: <n>\n- LEAK: <n>\n- TIME: <s>s\n- MIPS: <m>\n
+ * TIME is wall clock from boot to the result being read back.  A NULL
+ * buffer is rejected (the reference would run an uninitialised book). */
+void hvm_hip(uint32_t* book_buffer);
+
+/* ------------------------------------------------------------------------ */
+/* Structured entry                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hvm_hip_cfg {
+  uint64_t node_cap;    /* node slots in the net (0: 2^28)                    */
+  uint64_t vars_cap;    /* var slots in the net (0: 2^28)                     */
+  uint32_t bag_cap;     /* redex-bag entries per bag (0: 2^24)                */
+  uint32_t waves;       /* evaluator waves (0: fill every CU)                 */
+  uint32_t turn_steps;  /* steps per turn (0: adaptive)                       */
+  uint32_t flags;       /* HVM_HIP_NO_READBACK: skip the result text          */
+  uint32_t time_limit_s;/* abandon evaluation after this many seconds (0: none) */
+} hvm_hip_cfg;
+
+#define HVM_HIP_NO_READBACK 1u
+
+/* Interaction kinds, in the order of hvm_hip_out.kind[]. */
+enum {
+  HVM_HIP_K_CALL = 0, HVM_HIP_K_CALL_COPY = 1, HVM_HIP_K_VOID = 2, HVM_HIP_K_ERAS = 3,
+  HVM_HIP_K_ANNI = 4, HVM_HIP_K_COMM = 5, HVM_HIP_K_OPER1 = 6, HVM_HIP_K_OPER2 = 7,
+  HVM_HIP_K_SWIT0 = 8, HVM_HIP_K_SWITN = 9, HVM_HIP_KINDS = 10
+};
+
+typedef struct hvm_hip_out {
+  uint64_t itrs;                 /* ITRS, counted as src/hvm.c:1116-1118      */
+  uint64_t kind[HVM_HIP_KINDS];  /* per-kind interaction counts               */
+  uint64_t bytes;                /* algorithmic bytes of all interactions     */
+  uint64_t leak;                 /* LEAK line value (no local nets: 0)        */
+  uint64_t steps, spills, defers, retries, probes;  /* scheduler diagnostics  */
+  double kernel_ms;              /* sum of evaluator kernel durations         */
+  double eval_ms;                /* boot → normal form, incl. readback expansions */
+  double wall_s;                 /* boot → result read back (reference window) */
+  uint32_t turns;                /* evaluator launches                        */
+  uint32_t normalizes;           /* normalize passes (1 + readback expansions) */
+  uint32_t err;                  /* evaluator error flags (0 = none)          */
+  char* result;                  /* "Result:" text as the reference prints it */
+  char* canonical;               /* result with canonical var names (tests/run.rs:112-146) */
+} hvm_hip_out;
+
+/* Status codes */
+#define HVM_HIP_OK          0
+#define HVM_HIP_E_BOOK      1   /* book failed to load / exceeds limits       */
+#define HVM_HIP_E_IO        2   /* program requested IO (not supported here)  */
+#define HVM_HIP_E_DEVICE    3   /* HIP error or no GPU                        */
+#define HVM_HIP_E_EVAL      4   /* evaluator error (see out->err)             */
+#define HVM_HIP_E_ARG       5   /* bad argument                               */
+
+/* One-shot: create a context, load, evaluate, read back, destroy. */
+int hvm_hip_run(const uint32_t* book_buf, size_t n_words, const hvm_hip_cfg* cfg, hvm_hip_out* out);
+
+/* Frees result/canonical strings and anything else this library returns. */
+void hvm_hip_free(void* p);
+
+/* ------------------------------------------------------------------------ */
+/* Persistent context: the net, bags and book stay resident in HBM          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hvm_hip_ctx hvm_hip_ctx;
+
+hvm_hip_ctx* hvm_hip_create(const hvm_hip_cfg* cfg);          /* NULL on failure */
+int  hvm_hip_load(hvm_hip_ctx* ctx, const uint32_t* book_buf, size_t n_words);
+/* Clears the net, boots @main, normalises (with readback expansions) and,
+ * unless HVM_HIP_NO_READBACK, reads the result back. */
+int  hvm_hip_eval(hvm_hip_ctx* ctx, hvm_hip_out* out);
+void hvm_hip_destroy(hvm_hip_ctx* ctx);
+/* Last error message of a context (or of context creation when ctx is NULL). */
+const char* hvm_hip_last_error(const hvm_hip_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Front end (the reference's Rust host, restated in C++)                   */
+/* ------------------------------------------------------------------------ */
+
+/* Parses `.hvm` text and builds the wire buffer, as Book::parse + build +
+ * to_buffer do (src/ast.rs:528-559, src/hvm.rs:957-1003).  On success returns
+ * 0 and a malloc'd word array; on failure returns nonzero and a malloc'd
+ * message in *err (the reference panics with the same text). */
+int hvm_front_compile(const char* code, uint32_t** words, size_t* n_words, char** err);
+
+/* Canonical form of a printed tree: parse, rename vars x0, x1, … in
+ * first-occurrence order, re-print (tests/run.rs:112-146). */
+int hvm_front_canonical(const char* printed, char** out, char** err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

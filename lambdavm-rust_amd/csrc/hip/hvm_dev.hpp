@@ -1,0 +1,103 @@
+// hvm_dev.hpp — device-side layout shared by the evaluator kernels and the
+// host runtime.  Everything here is plain data: pointers and sizes.
+#pragma once
+#include <stdint.h>
+
+#include "../hvm_core.h"
+
+// ---------------------------------------------------------------------------
+// Tunables that shape the LDS footprint of one evaluator wave.
+// ---------------------------------------------------------------------------
+
+#define HVM_WAVE        64     // gfx950 wavefront width
+#define HVM_DEQ         32     // per-lane redex deque entries (power of two)
+#define HVM_TBL_NODE    512    // per-wave node-slot table (one step's allocations)
+#define HVM_TBL_VARS    512    // per-wave var-slot table
+
+// Largest def the evaluator instantiates in one interaction.  Defs are
+// checked at load time.
+#define HVM_GPU_DEF_MAX 512
+
+// ROOT is var 0x1FFFFFFF; the evaluator stores it in var slot 0, which the
+// allocators never hand out (src/hvm.c:717: slot 0 is skipped).
+#define HVM_ROOT_VAL 0x1FFFFFFFu
+
+// ---------------------------------------------------------------------------
+// Book as the device reads it: one record per fid plus a flat pair array.
+// ---------------------------------------------------------------------------
+
+struct HvmDevDef {
+  uint32_t rbag_off;   // index into pairs[] of the def's redexes
+  uint32_t node_off;   // index into pairs[] of the def's nodes
+  uint32_t rlen, nlen, vlen;
+  uint32_t root;
+  uint32_t safe;
+  uint32_t call_bytes; // 8N + 4V + 8(R+1), the canonical CALL byte count
+};
+
+// Statistic slots accumulated by the evaluator (same order as the oracle).
+enum {
+  HVM_K_CALL = 0, HVM_K_CALL_COPY, HVM_K_VOID, HVM_K_ERAS, HVM_K_ANNI, HVM_K_COMM,
+  HVM_K_OPER1, HVM_K_OPER2, HVM_K_SWIT0, HVM_K_SWITN, HVM_K_KINDS
+};
+
+struct HvmDevStats {
+  unsigned long long itrs;
+  unsigned long long bytes;
+  unsigned long long kind[HVM_K_KINDS];
+  unsigned long long steps;       // wave-steps executed
+  unsigned long long spills;      // redexes spilled from LDS deques to the HBM bag
+  unsigned long long defers;      // lane-steps deferred for allocation-table space
+  unsigned long long retries;     // interactions re-queued because a node was not yet visible
+  unsigned long long probes;      // 64-slot allocator windows scanned
+};
+
+// Error flags (bitwise) reported by the evaluator.
+enum {
+  HVM_ERR_OOM_NODE = 1, HVM_ERR_OOM_VARS = 2, HVM_ERR_BAG_OVERFLOW = 4,
+  HVM_ERR_LINK_LOOP = 8, HVM_ERR_BAD_DEF = 16, HVM_ERR_BAD_PORT = 32,
+};
+
+// Per-wave persistent allocator state (cursor into the wave's slices).
+struct HvmWaveState {
+  uint32_t node_cur;
+  uint32_t vars_cur;
+};
+
+struct HvmTurnArgs {
+  // net
+  unsigned long long* node;
+  uint32_t* vars;
+  uint32_t node_cap, vars_cap;     // slots (slot 0 never allocated)
+  uint32_t node_slice, vars_slice; // slots per wave
+  // book
+  const HvmDevDef* defs;
+  const unsigned long long* pairs;
+  uint32_t ndefs;
+  // bags
+  const unsigned long long* bag_in;
+  uint32_t n_in;
+  unsigned long long* bag_out;
+  uint32_t* n_out;                 // device counter
+  uint32_t* in_head;               // device counter for dynamic pulls
+  uint32_t bag_cap;
+  // control
+  uint32_t max_steps;              // steps per turn
+  uint32_t nwaves;
+  HvmWaveState* wave_state;
+  HvmDevStats* stats;
+  uint32_t* err;
+};
+
+struct HvmGatherArgs {
+  const unsigned long long* node;
+  const uint32_t* vars;
+  uint32_t node_cap, vars_cap;
+  uint32_t root_port;
+  // outputs: visited node indices/values and var indices/values
+  uint32_t* node_idx; unsigned long long* node_val; uint32_t* n_node;
+  uint32_t* var_idx; uint32_t* var_val; uint32_t* n_var;
+  uint32_t cap;                    // capacity of each output list
+  uint32_t* queue;                 // work queue of ports (cap entries)
+  uint32_t* err;
+};

@@ -1,0 +1,577 @@
+// hvm_eval.hip — the HVM2 evaluator for gfx950 (MI355X).
+//
+// One kernel launch is one *turn*.  Every wave (64 lanes) owns:
+//   * a redex deque per lane in LDS (HVM_DEQ entries, LIFO pop at the top,
+//     sharing from the bottom), fed from the HBM input bag;
+//   * a node slice and a var slice of the HBM net, from which it allocates
+//     with a wave-cooperative scan (one coalesced 64-slot load per window).
+// Each *step* every lane with a redex performs one interaction.  The rule code
+// runs as lockstep phases shared by all rules, so the 64 lanes issue their
+// memory operations together instead of serialising one rule at a time:
+//   P0 peek + classify (rule table, swap)       src/hvm.c:1073-1095
+//   P1 allocation-table reservation (wave scan)
+//   P2 wave-cooperative node/var allocation     replaces src/hvm.c:713-780
+//   P3 node takes (64-bit atomic exchange)      src/hvm.c:909-1069
+//   P4 node/var creation (device-scope stores)
+//   P5 one device-scope release fence per wave
+//   P6 links (32-bit exchanges on vars)         src/hvm.c:784-846
+//   P7 counting (ITRS += 1 per non-LINK rule)   src/hvm.c:1116-1118
+// Between steps, idle lanes take the oldest redex of a lane holding two or
+// more (in-wave sharing through LDS).  Redexes that overflow a deque, and all
+// redexes left at the end of a turn, go to the HBM output bag, which the host
+// hands to the next turn; a turn with an empty output bag ends normalisation.
+#include <hip/hip_runtime.h>
+
+#include "hvm_dev.hpp"
+
+typedef unsigned long long u64;
+typedef uint32_t u32;
+
+#define AGENT __HIP_MEMORY_SCOPE_AGENT
+
+// ---------------------------------------------------------------------------
+// Net accessors: every shared node/var access is a device-scope atomic.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ u32 vidx(u32 val) { return val == HVM_ROOT_VAL ? 0u : val; }
+
+// Every access is bounds-checked against the net's capacity: a port that
+// points outside the net raises HVM_ERR_BAD_PORT instead of touching memory
+// the net does not own.
+__device__ __forceinline__ u64 node_take(const HvmTurnArgs& A, u32 loc) {
+  if (loc >= A.node_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return 0; }
+  return __hip_atomic_exchange(&A.node[loc], 0ull, __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ void node_put(const HvmTurnArgs& A, u32 loc, u64 v) {
+  if (loc >= A.node_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return; }
+  __hip_atomic_store(&A.node[loc], v, __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ u32 vars_xchg(const HvmTurnArgs& A, u32 val, u32 x) {
+  u32 i = vidx(val);
+  if (i >= A.vars_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return HVM_NONE; }
+  return __hip_atomic_exchange(&A.vars[i], x, __ATOMIC_RELAXED, AGENT);
+}
+__device__ __forceinline__ void vars_put(const HvmTurnArgs& A, u32 val, u32 x) {
+  u32 i = vidx(val);
+  if (i >= A.vars_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return; }
+  __hip_atomic_store(&A.vars[i], x, __ATOMIC_RELAXED, AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Wave primitives
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ u32 lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ u32 rank_in(u64 mask) {
+  return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
+}
+
+// Exclusive prefix sum over the wave; *total receives the wave sum.
+__device__ __forceinline__ u32 wave_scan(u32 x, u32* total) {
+  u32 v = x;
+  u32 lane = lane_id();
+#pragma unroll
+  for (u32 d = 1; d < HVM_WAVE; d <<= 1) {
+    u32 t = __shfl_up(v, d, HVM_WAVE);
+    if (lane >= d) v += t;
+  }
+  *total = __shfl(v, HVM_WAVE - 1, HVM_WAVE);
+  return v - x;
+}
+
+__device__ __forceinline__ u32 wave_max(u32 x) {
+#pragma unroll
+  for (u32 d = 1; d < HVM_WAVE; d <<= 1) {
+    u32 t = __shfl_xor(x, d, HVM_WAVE);
+    x = x > t ? x : t;
+  }
+  return x;
+}
+
+__device__ __forceinline__ u64 wave_sum64(u64 x) {
+#pragma unroll
+  for (u32 d = 1; d < HVM_WAVE; d <<= 1) x += __shfl_xor(x, d, HVM_WAVE);
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// Per-wave LDS state
+// ---------------------------------------------------------------------------
+
+struct WaveLds {
+  u64 deq[HVM_DEQ * HVM_WAVE];  // deque entry j of lane l at [j*64 + l]
+  u32 tbl_n[HVM_TBL_NODE];
+  u32 tbl_v[HVM_TBL_VARS];
+  u64 inbox[HVM_WAVE];
+  u32 target[HVM_WAVE];
+  u32 cursor;                   // allocator scratch
+};
+
+struct Lane {
+  u32 bot, top;                 // deque ring counters
+  u32 kind[HVM_K_KINDS];
+  u64 bytes;
+  u32 spills;
+};
+
+__device__ __forceinline__ u64& deq_at(WaveLds& L, u32 idx) {
+  return L.deq[(idx & (HVM_DEQ - 1)) * HVM_WAVE + lane_id()];
+}
+
+// Pushes a redex on this lane's deque; when full, the oldest redex goes to
+// the HBM output bag.
+__device__ __forceinline__ void push_redex(WaveLds& L, Lane& ln, const HvmTurnArgs& A, u64 r) {
+  if (ln.top - ln.bot == HVM_DEQ) {
+    u64 old = deq_at(L, ln.bot);
+    ln.bot++;
+    u32 pos = atomicAdd(A.n_out, 1u);
+    if (pos < A.bag_cap) A.bag_out[pos] = old;
+    else atomicOr(A.err, (u32)HVM_ERR_BAG_OVERFLOW);
+    ln.spills++;
+  }
+  deq_at(L, ln.top) = r;
+  ln.top++;
+}
+
+// ---------------------------------------------------------------------------
+// Linker (src/hvm.c:796-842)
+// ---------------------------------------------------------------------------
+
+#define HVM_LINK_CAP (1u << 16)
+
+__device__ __forceinline__ u32 enter(const HvmTurnArgs& A, u32 var) {
+  for (u32 it = 0; it < HVM_LINK_CAP; ++it) {
+    if (hvm_get_tag(var) != HVM_VAR) return var;
+    u32 val = vars_xchg(A, hvm_get_val(var), HVM_NONE);
+    if (val == HVM_NONE || val == 0) return var;
+    vars_put(A, hvm_get_val(var), 0);
+    var = val;
+  }
+  atomicOr(A.err, (u32)HVM_ERR_LINK_LOOP);
+  return var;
+}
+
+__device__ __forceinline__ void link(WaveLds& L, Lane& ln, const HvmTurnArgs& A, u32 a, u32 b) {
+  for (u32 it = 0; it < HVM_LINK_CAP; ++it) {
+    if (hvm_get_tag(a) != HVM_VAR && hvm_get_tag(b) == HVM_VAR) { u32 x = a; a = b; b = x; }
+    if (hvm_get_tag(a) != HVM_VAR) { push_redex(L, ln, A, hvm_new_pair(a, b)); return; }
+    b = enter(A, b);
+    u32 a_ = vars_xchg(A, hvm_get_val(a), b);
+    if (a_ == HVM_NONE) return;
+    vars_put(A, hvm_get_val(a), 0);
+    a = a_;
+  }
+  atomicOr(A.err, (u32)HVM_ERR_LINK_LOOP);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-cooperative allocator: scans the wave's slice 64 slots at a time,
+// writing the first `need` free slot indices into `tbl`.  Only the owning
+// wave ever turns a zero slot of its slice non-zero, so a slot read as zero
+// is free (src/hvm.c:713-780 uses the same "zero means free" rule).
+// `need` and `cur` are wave-uniform.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ bool wave_alloc(WaveLds& L, T* buf, u32 lo, u32 len, u32& cur, u32 need,
+                                           u32* tbl, unsigned long long& probes) {
+  u32 lane = lane_id();
+  u32 got = 0;
+  u32 scanned = 0;
+  while (got < need) {
+    u32 rel = (cur - lo + lane) % len;
+    u32 idx = lo + rel;
+    T v = __hip_atomic_load(&buf[idx], __ATOMIC_RELAXED, AGENT);
+    bool free_ = (v == 0) && (idx != 0);
+    u64 m = __ballot(free_);
+    u32 r = rank_in(m);
+    u32 nfree = __popcll(m);
+    u32 want = need - got;
+    if (free_ && r < want) tbl[got + r] = idx;
+    if (nfree >= want) {
+      // next scan starts just after the last slot taken
+      if (free_ && r == want - 1) L.cursor = lo + ((rel + 1) % len);
+      __syncthreads();
+      cur = L.cursor;
+      got = need;
+    } else {
+      got += nfree;
+      cur = lo + ((cur - lo + HVM_WAVE) % len);
+    }
+    probes++;
+    scanned += HVM_WAVE;
+    if (got < need && scanned >= len + HVM_WAVE) return false;
+  }
+  __syncthreads();
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// The turn kernel
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
+  __shared__ WaveLds L;
+  const u32 lane = lane_id();
+  const u32 w = blockIdx.x;
+
+  Lane ln;
+  ln.bot = ln.top = 0;
+#pragma unroll
+  for (int k = 0; k < HVM_K_KINDS; ++k) ln.kind[k] = 0;
+  ln.bytes = 0;
+  ln.spills = 0;
+  u64 probes = 0, steps = 0, defers = 0, retries = 0;
+
+  const u32 n_lo = w * A.node_slice, v_lo = w * A.vars_slice;
+  u32 n_cur = A.wave_state[w].node_cur;
+  u32 v_cur = A.wave_state[w].vars_cur;
+  if (n_cur < n_lo || n_cur >= n_lo + A.node_slice) n_cur = n_lo;
+  if (v_cur < v_lo || v_cur >= v_lo + A.vars_slice) v_cur = v_lo;
+
+  // Initial spread: lane l of wave w takes item w + l*W, so a small bag is
+  // spread one redex per wave before any wave takes a second one.
+  {
+    u32 idx = w + lane * A.nwaves;
+    if (idx < A.n_in && idx < HVM_WAVE * A.nwaves) push_redex(L, ln, A, A.bag_in[idx]);
+  }
+  bool more = A.n_in > HVM_WAVE * A.nwaves;
+
+  for (u32 step = 0; step < A.max_steps; ++step) {
+    // ---- refill from the input bag ----
+    if (more) {
+      bool empty = ln.top == ln.bot;
+      u64 em = __ballot(empty);
+      if (em) {
+        u32 k = __popcll(em);
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(A.in_head, k);
+        base = __shfl(base, 0, HVM_WAVE);
+        if (base >= A.n_in) more = false;
+        u32 r = rank_in(em);
+        if (empty && base + r < A.n_in) push_redex(L, ln, A, A.bag_in[base + r]);
+      }
+    }
+
+    // ---- in-wave sharing: idle lanes take the oldest redex of rich lanes ----
+    {
+      u32 n = ln.top - ln.bot;
+      bool needy = n == 0, rich = n >= 2;
+      u64 nm = __ballot(needy), rm = __ballot(rich);
+      if (nm && rm) {
+        u32 k = min((u32)__popcll(nm), (u32)__popcll(rm));
+        u32 rn = rank_in(nm), rr = rank_in(rm);
+        if (needy && rn < k) L.target[rn] = lane;
+        __syncthreads();
+        if (rich && rr < k) {
+          u32 t = L.target[rr];
+          L.inbox[t] = deq_at(L, ln.bot);
+          ln.bot++;
+        }
+        __syncthreads();
+        if (needy && rn < k) push_redex(L, ln, A, L.inbox[lane]);
+      }
+    }
+
+    bool has = ln.top != ln.bot;
+    if (!__ballot(has)) {
+      if (!more) break;
+      continue;
+    }
+    if ((step & 63) == 0 && __hip_atomic_load(A.err, __ATOMIC_RELAXED, AGENT) != 0) break;
+    steps++;
+
+    // ---- P0: peek + classify ----
+    u64 rx = has ? deq_at(L, ln.top - 1) : 0ull;
+    u32 a = hvm_get_fst(rx), b = hvm_get_snd(rx);
+    u32 rule = hvm_get_rule(a, b);
+    if (hvm_get_tag(a) == HVM_REF && b == HVM_ROOT) rule = HVM_CALL;
+    else if (hvm_should_swap(a, b)) { u32 x = a; a = b; b = x; }
+    bool copy = false;
+    HvmDevDef d;
+    d.rlen = d.nlen = d.vlen = 0; d.rbag_off = d.node_off = 0; d.root = 0; d.safe = 0; d.call_bytes = 0;
+    u32 nn = 0, nv = 0;
+    if (has) {
+      if (rule == HVM_CALL) {
+        u32 fid = hvm_get_val(a) & 0xFFFFFFFu;
+        d = A.defs[fid < A.ndefs ? fid : 0];
+        if (d.safe && hvm_get_tag(b) == HVM_DUP) copy = true;
+        else { nn = d.nlen; nv = d.vlen; }
+      } else if (rule == HVM_COMM) { nn = 4; nv = 4; }
+      else if (rule == HVM_OPER) { nn = 1; }
+      else if (rule == HVM_SWIT) { nn = hvm_get_u24(hvm_get_val(a)) == 0 ? 1 : 2; }
+    }
+
+    // ---- P1: reserve allocation-table space (lanes that do not fit wait) ----
+    u32 tot_n, tot_v;
+    u32 off_n = wave_scan(nn, &tot_n);
+    u32 off_v = wave_scan(nv, &tot_v);
+    bool run = has && off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
+    if (has && !run) defers++;
+    u32 need_n = wave_max(run ? off_n + nn : 0);
+    u32 need_v = wave_max(run ? off_v + nv : 0);
+    if (run) ln.top--;
+
+    // ---- P2: allocation ----
+    if (need_n) {
+      if (!wave_alloc(L, A.node, n_lo, A.node_slice, n_cur, need_n, L.tbl_n, probes)) {
+        atomicOr(A.err, (u32)HVM_ERR_OOM_NODE);
+        if (run) ln.top++;
+        break;
+      }
+    }
+    if (need_v) {
+      if (!wave_alloc(L, A.vars, v_lo, A.vars_slice, v_cur, need_v, L.tbl_v, probes)) {
+        atomicOr(A.err, (u32)HVM_ERR_OOM_VARS);
+        if (run) ln.top++;
+        break;
+      }
+    }
+
+    // ---- P3: takes ----
+    bool take_a = run && (rule == HVM_ANNI || rule == HVM_COMM);
+    bool take_b = run && (rule == HVM_ERAS || rule == HVM_ANNI || rule == HVM_COMM ||
+                          rule == HVM_OPER || rule == HVM_SWIT || (rule == HVM_CALL && copy));
+    u64 NA = take_a ? node_take(A, hvm_get_val(a)) : 0ull;
+    u64 NB = take_b ? node_take(A, hvm_get_val(b)) : 0ull;
+    if ((take_a && NA == 0) || (take_b && NB == 0)) {
+      // A node this redex points to is not visible yet: put everything back
+      // and retry the redex on a later step (src/hvm.c:1112-1114).
+      if (take_a && NA) node_put(A, hvm_get_val(a), NA);
+      if (take_b && NB) node_put(A, hvm_get_val(b), NB);
+      ln.top++;
+      run = false;
+      retries++;
+    }
+
+    u32 B1 = hvm_get_fst(NB), B2 = hvm_get_snd(NB);
+    bool oper1 = false;
+    u32 cv = 0;
+    if (run && rule == HVM_OPER) {
+      B2 = enter(A, B2);
+      oper1 = hvm_get_tag(B1) == HVM_NUM;
+      if (oper1) cv = hvm_operate(hvm_get_val(a), hvm_get_val(B1));
+    }
+
+    // ---- P4: creation ----
+    const u32* tn = L.tbl_n + off_n;
+    const u32* tv = L.tbl_v + off_v;
+    if (run) {
+      if (rule == HVM_CALL && !copy) {
+        for (u32 i = 0; i < d.vlen; ++i) vars_put(A, tv[i], HVM_NONE);
+        for (u32 i = 0; i < d.nlen; ++i) {
+          u64 p = A.pairs[d.node_off + i];
+          u32 p1 = hvm_get_fst(p), p2 = hvm_get_snd(p);
+          if (hvm_is_nod(p1)) p1 = hvm_new_port(hvm_get_tag(p1), tn[hvm_get_val(p1)]);
+          else if (hvm_is_var(p1)) p1 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p1)]);
+          if (hvm_is_nod(p2)) p2 = hvm_new_port(hvm_get_tag(p2), tn[hvm_get_val(p2)]);
+          else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
+          node_put(A, tn[i], hvm_new_pair(p1, p2));
+        }
+      } else if (rule == HVM_COMM) {
+        vars_put(A, tv[0], HVM_NONE);
+        vars_put(A, tv[1], HVM_NONE);
+        vars_put(A, tv[2], HVM_NONE);
+        vars_put(A, tv[3], HVM_NONE);
+        node_put(A, tn[0], hvm_new_pair(hvm_new_port(HVM_VAR, tv[0]), hvm_new_port(HVM_VAR, tv[1])));
+        node_put(A, tn[1], hvm_new_pair(hvm_new_port(HVM_VAR, tv[2]), hvm_new_port(HVM_VAR, tv[3])));
+        node_put(A, tn[2], hvm_new_pair(hvm_new_port(HVM_VAR, tv[0]), hvm_new_port(HVM_VAR, tv[2])));
+        node_put(A, tn[3], hvm_new_pair(hvm_new_port(HVM_VAR, tv[1]), hvm_new_port(HVM_VAR, tv[3])));
+      } else if (rule == HVM_OPER && !oper1) {
+        node_put(A, tn[0], hvm_new_pair(a, B2));
+      } else if (rule == HVM_SWIT) {
+        u32 av = hvm_get_u24(hvm_get_val(a));
+        if (av == 0) {
+          node_put(A, tn[0], hvm_new_pair(B2, hvm_new_port(HVM_ERA, 0)));
+        } else {
+          node_put(A, tn[0], hvm_new_pair(hvm_new_port(HVM_ERA, 0), hvm_new_port(HVM_CON, tn[1])));
+          node_put(A, tn[1], hvm_new_pair(hvm_new_port(HVM_NUM, hvm_new_u24(av - 1)), B2));
+        }
+      }
+    }
+
+    // ---- P5: publish creations before any link can expose them ----
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+
+    // ---- P6: links ----
+    if (run) {
+      switch (rule) {
+        case HVM_LINK:
+          link(L, ln, A, a, b);
+          break;
+        case HVM_CALL:
+          if (copy) {
+            link(L, ln, A, a, B1);
+            link(L, ln, A, a, B2);
+          } else {
+            for (u32 i = 0; i < d.rlen; ++i) {
+              u64 p = A.pairs[d.rbag_off + i];
+              u32 p1 = hvm_get_fst(p), p2 = hvm_get_snd(p);
+              if (hvm_is_nod(p1)) p1 = hvm_new_port(hvm_get_tag(p1), tn[hvm_get_val(p1)]);
+              else if (hvm_is_var(p1)) p1 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p1)]);
+              if (hvm_is_nod(p2)) p2 = hvm_new_port(hvm_get_tag(p2), tn[hvm_get_val(p2)]);
+              else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
+              link(L, ln, A, p1, p2);
+            }
+            u32 r = d.root;
+            if (hvm_is_nod(r)) r = hvm_new_port(hvm_get_tag(r), tn[hvm_get_val(r)]);
+            else if (hvm_is_var(r)) r = hvm_new_port(HVM_VAR, tv[hvm_get_val(r)]);
+            link(L, ln, A, r, b);
+          }
+          break;
+        case HVM_ERAS:
+          link(L, ln, A, a, B1);
+          link(L, ln, A, a, B2);
+          break;
+        case HVM_ANNI:
+          link(L, ln, A, hvm_get_fst(NA), B1);
+          link(L, ln, A, hvm_get_snd(NA), B2);
+          break;
+        case HVM_COMM:
+          link(L, ln, A, hvm_new_port(hvm_get_tag(b), tn[0]), hvm_get_fst(NA));
+          link(L, ln, A, hvm_new_port(hvm_get_tag(b), tn[1]), hvm_get_snd(NA));
+          link(L, ln, A, hvm_new_port(hvm_get_tag(a), tn[2]), B1);
+          link(L, ln, A, hvm_new_port(hvm_get_tag(a), tn[3]), B2);
+          break;
+        case HVM_OPER:
+          if (oper1) link(L, ln, A, hvm_new_port(HVM_NUM, cv), B2);
+          else link(L, ln, A, B1, hvm_new_port(HVM_OPR, tn[0]));
+          break;
+        case HVM_SWIT:
+          link(L, ln, A, hvm_new_port(HVM_CON, tn[0]), B1);
+          break;
+        default:
+          break;
+      }
+
+      // ---- P7: counting ----
+      switch (rule) {
+        case HVM_CALL:
+          if (copy) { ln.kind[HVM_K_CALL_COPY]++; ln.bytes += 32; }
+          else { ln.kind[HVM_K_CALL]++; ln.bytes += d.call_bytes; }
+          break;
+        case HVM_VOID: ln.kind[HVM_K_VOID]++; break;
+        case HVM_ERAS: ln.kind[HVM_K_ERAS]++; ln.bytes += 32; break;
+        case HVM_ANNI: ln.kind[HVM_K_ANNI]++; ln.bytes += 48; break;
+        case HVM_COMM: ln.kind[HVM_K_COMM]++; ln.bytes += 112; break;
+        case HVM_OPER:
+          if (oper1) { ln.kind[HVM_K_OPER1]++; ln.bytes += 32; }
+          else { ln.kind[HVM_K_OPER2]++; ln.bytes += 40; }
+          break;
+        case HVM_SWIT:
+          if (hvm_get_u24(hvm_get_val(a)) == 0) { ln.kind[HVM_K_SWIT0]++; ln.bytes += 32; }
+          else { ln.kind[HVM_K_SWITN]++; ln.bytes += 40; }
+          break;
+        default: break;
+      }
+    }
+  }
+
+  // ---- end of turn: everything left goes to the output bag ----
+  {
+    u32 n = ln.top - ln.bot;
+    u32 tot;
+    u32 off = wave_scan(n, &tot);
+    u32 base = 0;
+    if (lane == 0 && tot) base = atomicAdd(A.n_out, tot);
+    base = __shfl(base, 0, HVM_WAVE);
+    if (base + tot > A.bag_cap) {
+      if (lane == 0) atomicOr(A.err, (u32)HVM_ERR_BAG_OVERFLOW);
+    } else {
+      for (u32 i = 0; i < n; ++i) A.bag_out[base + off + i] = deq_at(L, ln.bot + i);
+    }
+  }
+
+  // ---- statistics: one atomic per counter per wave ----
+  {
+    u64 itrs = 0;
+#pragma unroll
+    for (int k = 0; k < HVM_K_KINDS; ++k) itrs += ln.kind[k];
+    u64 s_itrs = wave_sum64(itrs);
+    u64 s_bytes = wave_sum64(ln.bytes);
+    u64 s_spill = wave_sum64(ln.spills);
+    u64 kinds[HVM_K_KINDS];
+#pragma unroll
+    for (int k = 0; k < HVM_K_KINDS; ++k) kinds[k] = wave_sum64(ln.kind[k]);
+    if (lane == 0) {
+      if (s_itrs) atomicAdd(&A.stats->itrs, s_itrs);
+      if (s_bytes) atomicAdd(&A.stats->bytes, s_bytes);
+      if (s_spill) atomicAdd(&A.stats->spills, s_spill);
+#pragma unroll
+      for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&A.stats->kind[k], kinds[k]);
+      if (steps) atomicAdd(&A.stats->steps, steps);
+      if (defers) atomicAdd(&A.stats->defers, defers);
+      if (retries) atomicAdd(&A.stats->retries, retries);
+      if (probes) atomicAdd(&A.stats->probes, probes);
+      A.wave_state[w].node_cur = n_cur;
+      A.wave_state[w].vars_cur = v_cur;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Readback: level-synchronous traversal from a port, recording every node and
+// var it reaches, for the host printer.  One workgroup; bounded by `cap`.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void hvm_gather_kernel(HvmGatherArgs G) {
+  __shared__ u32 s_head, s_tail, s_next;
+  if (threadIdx.x == 0) {
+    G.queue[0] = G.root_port;
+    s_head = 0;
+    s_tail = 1;
+    s_next = 1;
+  }
+  __syncthreads();
+  for (u32 round = 0; round < (1u << 30); ++round) {
+    u32 head = s_head, tail = s_tail;
+    if (head >= tail) break;
+    for (u32 i = head + threadIdx.x; i < tail; i += blockDim.x) {
+      u32 p = G.queue[i];
+      u32 tag = hvm_get_tag(p);
+      if (tag == HVM_VAR) {
+        u32 vi = vidx(hvm_get_val(p));
+        if (vi >= G.vars_cap) { atomicOr(G.err, 1u); continue; }
+        u32 v = G.vars[vi];
+        u32 k = atomicAdd(G.n_var, 1u);
+        if (k < G.cap) { G.var_idx[k] = hvm_get_val(p); G.var_val[k] = v; }
+        if (v != HVM_NONE && v != 0) {
+          u32 q = atomicAdd(&s_next, 1u);
+          if (q < G.cap) G.queue[q] = v; else atomicOr(G.err, 2u);
+        }
+      } else if (tag >= HVM_CON) {
+        u32 ni = hvm_get_val(p);
+        if (ni >= G.node_cap) { atomicOr(G.err, 1u); continue; }
+        u64 nd = G.node[ni];
+        u32 k = atomicAdd(G.n_node, 1u);
+        if (k < G.cap) { G.node_idx[k] = ni; G.node_val[k] = nd; }
+        u32 q = atomicAdd(&s_next, 2u);
+        if (q + 1 < G.cap) { G.queue[q] = hvm_get_fst(nd); G.queue[q + 1] = hvm_get_snd(nd); }
+        else atomicOr(G.err, 2u);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_head = tail;
+      s_tail = s_next < G.cap ? s_next : G.cap;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers (host)
+// ---------------------------------------------------------------------------
+
+extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s) {
+  hipLaunchKernelGGL(hvm_turn_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s) {
+  hipLaunchKernelGGL(hvm_gather_kernel, dim3(1), dim3(256), 0, s, *g);
+  return hipGetLastError();
+}
+
+extern "C" size_t hvm_turn_lds_bytes(void) { return sizeof(WaveLds); }

@@ -1,0 +1,597 @@
+// hvm_runtime.hip — host side of the MI355X evaluator: net and book in HBM,
+// the turn loop, the run-cu IO driver's readback, result printing, and the
+// C ABI declared in include/hvm_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/hvm_hip.h"
+#include "../front/hvm_front.hpp"
+#include "hvm_dev.hpp"
+
+extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s);
+extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s);
+extern "C" size_t hvm_turn_lds_bytes(void);
+
+namespace {
+
+typedef unsigned long long u64;
+typedef uint32_t u32;
+
+static std::string g_create_error;
+
+static char* dup_cstr(const std::string& s) {
+  char* p = (char*)malloc(s.size() + 1);
+  memcpy(p, s.c_str(), s.size() + 1);
+  return p;
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct hvm_hip_ctx {
+  hvm_hip_cfg cfg{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int cus = 0;
+  // net
+  u64* node = nullptr;
+  u32* vars = nullptr;
+  u32 node_cap = 0, vars_cap = 0;
+  u32 nwaves = 0, node_slice = 0, vars_slice = 0;
+  // bags + control
+  u64* bag[2] = {nullptr, nullptr};
+  u32 bag_cap = 0;
+  u32* ctl = nullptr;          // [0] n_out, [1] in_head, [2] err
+  HvmWaveState* wave_state = nullptr;
+  HvmDevStats* stats = nullptr;
+  // book
+  hvm::Book book;
+  HvmDevDef* d_defs = nullptr;
+  u64* d_pairs = nullptr;
+  u32 ndefs = 0;
+  bool loaded = false;
+  // per-eval bookkeeping
+  double kernel_ms = 0;
+  u32 turns = 0, normalizes = 0;
+  std::string last_error;
+
+  bool fail(const std::string& m) { last_error = m; return false; }
+  bool ck(hipError_t e, const char* what) {
+    if (e != hipSuccess) return fail(std::string(what) + ": " + hipGetErrorString(e));
+    return true;
+  }
+
+  bool init(const hvm_hip_cfg* c) {
+    if (c) cfg = *c;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("no HIP device");
+    if (!ck(hipSetDevice(0), "hipSetDevice")) return false;
+    hipDeviceProp_t prop;
+    if (!ck(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties")) return false;
+    cus = prop.multiProcessorCount;
+    uint64_t ncap = cfg.node_cap ? cfg.node_cap : (1ull << 28);
+    uint64_t vcap = cfg.vars_cap ? cfg.vars_cap : (1ull << 28);
+    if (ncap > (1ull << 29) || vcap > (1ull << 29)) return fail("node_cap/vars_cap exceed 2^29");
+    node_cap = (u32)ncap;
+    vars_cap = (u32)vcap;
+    bag_cap = cfg.bag_cap ? cfg.bag_cap : (1u << 24);
+    if (cfg.waves) {
+      nwaves = cfg.waves;
+    } else {
+      // Fill the chip: as many one-wave workgroups as the LDS allows per CU.
+      size_t lds = hvm_turn_lds_bytes();
+      size_t per_cu = (160u * 1024u) / (lds ? lds : 1);
+      if (per_cu < 1) per_cu = 1;
+      if (per_cu > 16) per_cu = 16;
+      nwaves = (u32)(cus * per_cu);
+    }
+    node_slice = node_cap / nwaves;
+    vars_slice = vars_cap / nwaves;
+    if (node_slice < 4 * HVM_WAVE || vars_slice < 4 * HVM_WAVE) return fail("net too small for the wave count");
+    if (!ck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+    if (!ck(hipEventCreate(&ev0), "hipEventCreate") || !ck(hipEventCreate(&ev1), "hipEventCreate")) return false;
+    if (!ck(hipMalloc(&node, (size_t)node_cap * 8), "hipMalloc(node)")) return false;
+    if (!ck(hipMalloc(&vars, (size_t)vars_cap * 4), "hipMalloc(vars)")) return false;
+    if (!ck(hipMalloc(&bag[0], (size_t)bag_cap * 8), "hipMalloc(bag)")) return false;
+    if (!ck(hipMalloc(&bag[1], (size_t)bag_cap * 8), "hipMalloc(bag)")) return false;
+    if (!ck(hipMalloc(&ctl, 16 * sizeof(u32)), "hipMalloc(ctl)")) return false;
+    if (!ck(hipMalloc(&wave_state, nwaves * sizeof(HvmWaveState)), "hipMalloc(wave_state)")) return false;
+    if (!ck(hipMalloc(&stats, sizeof(HvmDevStats)), "hipMalloc(stats)")) return false;
+    return true;
+  }
+
+  void release() {
+    if (node) hipFree(node);
+    if (vars) hipFree(vars);
+    if (bag[0]) hipFree(bag[0]);
+    if (bag[1]) hipFree(bag[1]);
+    if (ctl) hipFree(ctl);
+    if (wave_state) hipFree(wave_state);
+    if (stats) hipFree(stats);
+    if (d_defs) hipFree(d_defs);
+    if (d_pairs) hipFree(d_pairs);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  bool load(const u32* buf, size_t n_words) {
+    std::string err;
+    hvm::Book b;
+    if (!buf) return fail("failed to load book: NULL buffer");
+    if (!hvm::load_buffer(buf, n_words, b, err)) return fail(err);
+    if (b.defs.empty()) return fail("failed to load book: no defs");
+    std::vector<HvmDevDef> defs(b.defs.size());
+    std::vector<u64> pairs;
+    for (size_t f = 0; f < b.defs.size(); ++f) {
+      const hvm::Def& d = b.defs[f];
+      if (d.node.size() > HVM_GPU_DEF_MAX || d.vars > HVM_GPU_DEF_MAX)
+        return fail("def '" + d.name + "' exceeds the evaluator's per-def limit of " +
+                    std::to_string(HVM_GPU_DEF_MAX) + " nodes/vars");
+      // Every template port must stay inside its def, so that instantiation
+      // only touches slots the allocator handed out.
+      auto port_ok = [&](u32 p) {
+        hvm_tag t = hvm_get_tag(p);
+        u32 v = hvm_get_val(p);
+        if (t >= HVM_CON) return v < d.node.size();
+        if (t == HVM_VAR) return v < d.vars;
+        if (t == HVM_REF) return (v & 0xFFFFFFFu) < b.defs.size();
+        return true;
+      };
+      bool ok = port_ok(d.root);
+      for (u64 p : d.rbag) ok = ok && port_ok(hvm_get_fst(p)) && port_ok(hvm_get_snd(p));
+      for (u64 p : d.node) ok = ok && port_ok(hvm_get_fst(p)) && port_ok(hvm_get_snd(p));
+      if (!ok) return fail("def '" + d.name + "' has a port outside its own nodes/vars");
+      HvmDevDef& x = defs[f];
+      x.rbag_off = (u32)pairs.size();
+      pairs.insert(pairs.end(), d.rbag.begin(), d.rbag.end());
+      x.node_off = (u32)pairs.size();
+      pairs.insert(pairs.end(), d.node.begin(), d.node.end());
+      x.rlen = (u32)d.rbag.size();
+      x.nlen = (u32)d.node.size();
+      x.vlen = d.vars;
+      x.root = d.root;
+      x.safe = d.safe ? 1 : 0;
+      x.call_bytes = 8 * x.nlen + 4 * x.vlen + 8 * (x.rlen + 1);
+    }
+    if (pairs.empty()) pairs.push_back(0);
+    if (d_defs) { hipFree(d_defs); d_defs = nullptr; }
+    if (d_pairs) { hipFree(d_pairs); d_pairs = nullptr; }
+    if (!ck(hipMalloc(&d_defs, defs.size() * sizeof(HvmDevDef)), "hipMalloc(defs)")) return false;
+    if (!ck(hipMalloc(&d_pairs, pairs.size() * 8), "hipMalloc(pairs)")) return false;
+    if (!ck(hipMemcpy(d_defs, defs.data(), defs.size() * sizeof(HvmDevDef), hipMemcpyHostToDevice), "copy defs")) return false;
+    if (!ck(hipMemcpy(d_pairs, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice), "copy pairs")) return false;
+    ndefs = (u32)defs.size();
+    book = std::move(b);
+    loaded = true;
+    return true;
+  }
+
+  // --- small host accessors (the reference's gnet_* helpers, src/hvm.cu:1842-1881)
+  u64 node_load(u32 loc) {
+    u64 v = 0;
+    if (loc < node_cap) hipMemcpy(&v, node + loc, 8, hipMemcpyDeviceToHost);
+    return v;
+  }
+  u32 vidx(u32 val) const { return val == HVM_ROOT_VAL ? 0u : val; }
+  u32 vars_load(u32 val) {
+    u32 v = 0;
+    u32 i = vidx(val);
+    if (i < vars_cap) hipMemcpy(&v, vars + i, 4, hipMemcpyDeviceToHost);
+    return v;
+  }
+  void vars_store(u32 val, u32 x) {
+    u32 i = vidx(val);
+    if (i < vars_cap) hipMemcpy(vars + i, &x, 4, hipMemcpyHostToDevice);
+  }
+  u32 peek(u32 port) {
+    for (int it = 0; it < (1 << 24) && hvm_get_tag(port) == HVM_VAR; ++it) {
+      u32 v = vars_load(hvm_get_val(port));
+      if (v == HVM_NONE || v == 0) break;
+      port = v;
+    }
+    return port;
+  }
+
+  bool reset_net() {
+    if (!ck(hipMemsetAsync(node, 0, (size_t)node_cap * 8, stream), "clear node")) return false;
+    if (!ck(hipMemsetAsync(vars, 0, (size_t)vars_cap * 4, stream), "clear vars")) return false;
+    if (!ck(hipMemsetAsync(wave_state, 0, nwaves * sizeof(HvmWaveState), stream), "clear cursors")) return false;
+    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return false;
+    if (!ck(hipMemsetAsync(ctl, 0, 16 * sizeof(u32), stream), "clear ctl")) return false;
+    return ck(hipStreamSynchronize(stream), "reset sync");
+  }
+
+  // Boots `redex` (src/hvm.c:1204-1207): ROOT := NONE, bag := [redex].
+  bool boot(u64 redex) {
+    u32 none = HVM_NONE;
+    if (!ck(hipMemcpyAsync(vars, &none, 4, hipMemcpyHostToDevice, stream), "boot root")) return false;
+    if (!ck(hipMemcpyAsync(bag[0], &redex, 8, hipMemcpyHostToDevice, stream), "boot redex")) return false;
+    pending = 1;
+    cur = 0;
+    return ck(hipStreamSynchronize(stream), "boot sync");
+  }
+
+  u32 pending = 0;  // redexes in bag[cur]
+  int cur = 0;
+
+  u32 turn_steps(u32 n) const {
+    if (cfg.turn_steps) return cfg.turn_steps;
+    u64 lanes = (u64)nwaves * HVM_WAVE;
+    if (n < nwaves) return 24;
+    if (n < lanes / 4) return 96;
+    return 4096;
+  }
+
+  // The turn loop: replaces gnet_normalize (src/hvm.cu:1810-1839).
+  double deadline = 0;  // steady-clock seconds; 0 = no limit
+
+  bool normalize() {
+    normalizes++;
+    while (pending > 0) {
+      if (deadline > 0 && now_s() > deadline) return fail("time limit exceeded");
+      HvmTurnArgs a;
+      a.node = node; a.vars = vars;
+      a.node_cap = node_cap; a.vars_cap = vars_cap;
+      a.node_slice = node_slice; a.vars_slice = vars_slice;
+      a.defs = d_defs; a.pairs = d_pairs; a.ndefs = ndefs;
+      a.bag_in = bag[cur]; a.n_in = pending;
+      a.bag_out = bag[cur ^ 1];
+      a.n_out = ctl + 0; a.in_head = ctl + 1; a.err = ctl + 2;
+      a.bag_cap = bag_cap;
+      a.max_steps = turn_steps(pending);
+      a.nwaves = nwaves;
+      a.wave_state = wave_state;
+      a.stats = stats;
+      u32 init[2] = {0, (u32)std::min<u64>((u64)nwaves * HVM_WAVE, pending)};
+      if (!ck(hipMemcpyAsync(ctl, init, 8, hipMemcpyHostToDevice, stream), "turn init")) return false;
+      hipEventRecord(ev0, stream);
+      if (!ck(hvm_launch_turn(&a, stream), "launch turn")) return false;
+      hipEventRecord(ev1, stream);
+      u32 back[3];
+      if (!ck(hipMemcpyAsync(back, ctl, 12, hipMemcpyDeviceToHost, stream), "turn readback")) return false;
+      if (!ck(hipStreamSynchronize(stream), "turn sync")) return false;
+      float ms = 0;
+      hipEventElapsedTime(&ms, ev0, ev1);
+      kernel_ms += ms;
+      turns++;
+      if (back[2]) return fail("evaluator error flags 0x" + std::to_string(back[2]));
+      u32 n_out = back[0], head = back[1];
+      if (n_out > bag_cap) return fail("redex bag overflow");
+      if (head < pending) {
+        // Redexes of the input bag that no wave pulled this turn carry over.
+        u32 rem = pending - head;
+        if ((u64)n_out + rem > bag_cap) return fail("redex bag overflow");
+        if (!ck(hipMemcpyAsync(bag[cur ^ 1] + n_out, bag[cur] + head, (size_t)rem * 8,
+                               hipMemcpyDeviceToDevice, stream), "carry")) return false;
+        n_out += rem;
+      }
+      pending = n_out;
+      cur ^= 1;
+    }
+    return true;
+  }
+
+  // src/hvm.c:1233-1244 / src/hvm.cu:1871-1881
+  bool expand(u32 port, u32& got) {
+    u32 old = vars_load(HVM_ROOT_VAL);
+    got = peek(port);
+    while (hvm_get_tag(got) == HVM_REF) {
+      if (!boot(hvm_new_pair(got, HVM_ROOT))) return false;
+      if (!normalize()) return false;
+      got = peek(vars_load(HVM_ROOT_VAL));
+    }
+    vars_store(HVM_ROOT_VAL, old);
+    return true;
+  }
+
+  struct Ctr { int32_t tag = -1; u32 args_len = 0; u32 args[16]; };
+
+  // src/run.cu:67-100 (readback_ctr).  Arguments past the 16th are expanded
+  // (as the reference does) but not stored.
+  bool readback_ctr(u32 port, Ctr& c) {
+    u32 lam;
+    if (!expand(port, lam)) return false;
+    if (hvm_get_tag(lam) != HVM_CON) return true;
+    u64 lam_node = node_load(hvm_get_val(lam));
+    u32 app;
+    if (!expand(hvm_get_fst(lam_node), app)) return false;
+    if (hvm_get_tag(app) != HVM_CON) return true;
+    u64 app_node = node_load(hvm_get_val(app));
+    u32 arg;
+    if (!expand(hvm_get_fst(app_node), arg)) return false;
+    if (hvm_get_tag(arg) != HVM_NUM) return true;
+    c.tag = (int32_t)hvm_get_u24(hvm_get_val(arg));
+    for (u32 it = 0; it < (1u << 24); ++it) {
+      if (!expand(hvm_get_snd(app_node), app)) return false;
+      if (hvm_get_tag(app) != HVM_CON) break;
+      app_node = node_load(hvm_get_val(app));
+      if (!expand(hvm_get_fst(app_node), arg)) return false;
+      if (c.args_len < 16) c.args[c.args_len] = arg;
+      c.args_len++;
+    }
+    return true;
+  }
+
+  // src/run.cu:857-925 without the IO calls: returns 2 when the program asks
+  // for an IO call (out of scope for this evaluator).
+  int run_io() {
+    if (!normalize()) return HVM_HIP_E_EVAL;
+    Ctr c;
+    if (!readback_ctr(peek(HVM_ROOT), c)) return HVM_HIP_E_EVAL;
+    if (c.args_len < 1 || hvm_get_tag(c.args[0]) != HVM_CON) return 0;
+    u64 magic = node_load(hvm_get_val(c.args[0]));
+    if (hvm_get_val(hvm_get_fst(magic)) != hvm_new_u24(0xD0CA11) ||
+        hvm_get_val(hvm_get_snd(magic)) != hvm_new_u24(0xFF1FF1)) return 0;
+    if (c.tag == 1 && c.args_len == 4) return HVM_HIP_E_IO;
+    return 0;
+  }
+
+  // Reads back everything reachable from `port` and prints it the way
+  // pretty_print_port does (src/hvm.c:1618-1699).
+  bool print_result(u32 port, std::string& out) {
+    u32 cap = 1u << 20;
+    for (int attempt = 0; attempt < 6; ++attempt, cap <<= 2) {
+      u32 *d_nidx = nullptr, *d_vidx = nullptr, *d_vval = nullptr, *d_q = nullptr, *d_cnt = nullptr;
+      u64* d_nval = nullptr;
+      bool ok = ck(hipMalloc(&d_nidx, (size_t)cap * 4), "gather alloc") &&
+                ck(hipMalloc(&d_nval, (size_t)cap * 8), "gather alloc") &&
+                ck(hipMalloc(&d_vidx, (size_t)cap * 4), "gather alloc") &&
+                ck(hipMalloc(&d_vval, (size_t)cap * 4), "gather alloc") &&
+                ck(hipMalloc(&d_q, (size_t)cap * 4), "gather alloc") &&
+                ck(hipMalloc(&d_cnt, 16), "gather alloc");
+      u32 cnt[3] = {0, 0, 0};
+      std::vector<u32> nidx, vidx_, vval;
+      std::vector<u64> nval;
+      if (ok) {
+        hipMemsetAsync(d_cnt, 0, 16, stream);
+        HvmGatherArgs g;
+        g.node = node; g.vars = vars; g.node_cap = node_cap; g.vars_cap = vars_cap;
+        g.root_port = port;
+        g.node_idx = d_nidx; g.node_val = d_nval; g.n_node = d_cnt;
+        g.var_idx = d_vidx; g.var_val = d_vval; g.n_var = d_cnt + 1;
+        g.cap = cap; g.queue = d_q; g.err = d_cnt + 2;
+        ok = ck(hvm_launch_gather(&g, stream), "launch gather") &&
+             ck(hipMemcpyAsync(cnt, d_cnt, 12, hipMemcpyDeviceToHost, stream), "gather counts") &&
+             ck(hipStreamSynchronize(stream), "gather sync");
+        if (ok && !(cnt[2] & 2) && cnt[0] <= cap && cnt[1] <= cap) {
+          nidx.resize(cnt[0]); nval.resize(cnt[0]); vidx_.resize(cnt[1]); vval.resize(cnt[1]);
+          ok = ck(hipMemcpy(nidx.data(), d_nidx, (size_t)cnt[0] * 4, hipMemcpyDeviceToHost), "gather copy") &&
+               ck(hipMemcpy(nval.data(), d_nval, (size_t)cnt[0] * 8, hipMemcpyDeviceToHost), "gather copy") &&
+               ck(hipMemcpy(vidx_.data(), d_vidx, (size_t)cnt[1] * 4, hipMemcpyDeviceToHost), "gather copy") &&
+               ck(hipMemcpy(vval.data(), d_vval, (size_t)cnt[1] * 4, hipMemcpyDeviceToHost), "gather copy");
+        }
+      }
+      hipFree(d_nidx); hipFree(d_nval); hipFree(d_vidx); hipFree(d_vval); hipFree(d_q); hipFree(d_cnt);
+      if (!ok) return false;
+      if (cnt[2] & 1) return fail("readback reached a port outside the net");
+      if (cnt[2] & 2 || cnt[0] > cap || cnt[1] > cap) continue;  // grow and retry
+      std::unordered_map<u32, u64> nodes;
+      std::unordered_map<u32, u32> varv;
+      nodes.reserve(nidx.size() * 2);
+      for (size_t i = 0; i < nidx.size(); ++i) nodes[nidx[i]] = nval[i];
+      for (size_t i = 0; i < vidx_.size(); ++i) varv[vidx_[i]] = vval[i];
+      out.clear();
+      std::vector<u32> st{port};
+      while (!st.empty()) {
+        u32 c = st.back();
+        st.pop_back();
+        switch (hvm_get_tag(c)) {
+          case HVM_CON: case HVM_DUP: case HVM_OPR: case HVM_SWI: {
+            u64 nd = nodes.count(hvm_get_val(c)) ? nodes[hvm_get_val(c)] : 0;
+            hvm_tag t = hvm_get_tag(c);
+            out += t == HVM_CON ? "(" : t == HVM_DUP ? "{" : t == HVM_OPR ? "$(" : "?(";
+            st.push_back(hvm_new_port(HVM_ERA, t == HVM_DUP ? '}' : ')'));
+            st.push_back(hvm_get_snd(nd));
+            st.push_back(hvm_new_port(HVM_ERA, ' '));
+            st.push_back(hvm_get_fst(nd));
+            break;
+          }
+          case HVM_ERA:
+            if (hvm_get_val(c)) out += (char)hvm_get_val(c); else out += "*";
+            break;
+          case HVM_VAR: {
+            auto it = varv.find(hvm_get_val(c));
+            u32 got = it == varv.end() ? HVM_NONE : it->second;
+            if (got != HVM_NONE) {
+              st.push_back(got);
+            } else {
+              char b[16];
+              snprintf(b, sizeof b, "x%x", hvm_get_val(c));
+              out += b;
+            }
+            break;
+          }
+          case HVM_NUM: out += hvm::c_show_numb(hvm_get_val(c)); break;
+          case HVM_REF: {
+            u32 fid = hvm_get_val(c) & 0xFFFFFFFu;
+            out += "@" + (fid < book.defs.size() ? book.defs[fid].name : std::string("?"));
+            break;
+          }
+        }
+      }
+      return true;
+    }
+    return fail("result too large to read back");
+  }
+
+  int eval(hvm_hip_out* out) {
+    if (!loaded) { fail("no book loaded"); return HVM_HIP_E_ARG; }
+    kernel_ms = 0;
+    turns = 0;
+    normalizes = 0;
+    if (!reset_net()) return HVM_HIP_E_DEVICE;
+    double t0 = now_s();
+    deadline = cfg.time_limit_s ? t0 + cfg.time_limit_s : 0;
+    if (!boot(hvm_new_pair(hvm_new_port(HVM_REF, 0), HVM_ROOT))) return HVM_HIP_E_DEVICE;
+    int rc = run_io();
+    double t_eval = now_s();
+    HvmDevStats s;
+    memset(&s, 0, sizeof s);
+    hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost);
+    if (out) {
+      memset(out, 0, sizeof *out);
+      out->itrs = s.itrs;
+      for (int k = 0; k < HVM_HIP_KINDS; ++k) out->kind[k] = s.kind[k];
+      out->bytes = s.bytes;
+      out->steps = s.steps; out->spills = s.spills; out->defers = s.defers;
+      out->retries = s.retries; out->probes = s.probes;
+      out->kernel_ms = kernel_ms;
+      out->eval_ms = (t_eval - t0) * 1e3;
+      out->turns = turns;
+      out->normalizes = normalizes;
+    }
+    if (rc != 0) {
+      u32 e = 0;
+      hipMemcpy(&e, ctl + 2, 4, hipMemcpyDeviceToHost);
+      if (out) out->err = e;
+      return rc;
+    }
+    if (!(cfg.flags & HVM_HIP_NO_READBACK)) {
+      std::string text;
+      if (!print_result(peek(HVM_ROOT), text)) return HVM_HIP_E_EVAL;
+      double t1 = now_s();
+      if (out) {
+        out->wall_s = t1 - t0;
+        out->result = dup_cstr(text);
+        std::string canon;
+        try { canon = hvm::canonical_tree_text(text); } catch (std::exception& ex) { canon = "<unparsable: " + std::string(ex.what()) + ">"; }
+        out->canonical = dup_cstr(canon);
+      }
+    } else if (out) {
+      out->wall_s = t_eval - t0;
+    }
+    return HVM_HIP_OK;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" hvm_hip_ctx* hvm_hip_create(const hvm_hip_cfg* cfg) {
+  hvm_hip_ctx* c = new hvm_hip_ctx();
+  if (!c->init(cfg)) {
+    g_create_error = c->last_error;
+    c->release();
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+extern "C" int hvm_hip_load(hvm_hip_ctx* ctx, const uint32_t* book_buf, size_t n_words) {
+  if (!ctx) return HVM_HIP_E_ARG;
+  return ctx->load(book_buf, n_words) ? HVM_HIP_OK : HVM_HIP_E_BOOK;
+}
+
+extern "C" int hvm_hip_eval(hvm_hip_ctx* ctx, hvm_hip_out* out) {
+  if (!ctx) return HVM_HIP_E_ARG;
+  return ctx->eval(out);
+}
+
+extern "C" void hvm_hip_destroy(hvm_hip_ctx* ctx) {
+  if (!ctx) return;
+  ctx->release();
+  delete ctx;
+}
+
+extern "C" const char* hvm_hip_last_error(const hvm_hip_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : g_create_error.c_str();
+}
+
+extern "C" void hvm_hip_free(void* p) { free(p); }
+
+extern "C" int hvm_hip_run(const uint32_t* book_buf, size_t n_words, const hvm_hip_cfg* cfg, hvm_hip_out* out) {
+  if (!book_buf) return HVM_HIP_E_ARG;
+  hvm_hip_ctx* c = hvm_hip_create(cfg);
+  if (!c) return HVM_HIP_E_DEVICE;
+  int rc = c->load(book_buf, n_words) ? HVM_HIP_OK : HVM_HIP_E_BOOK;
+  if (rc == HVM_HIP_OK) rc = c->eval(out);
+  hvm_hip_destroy(c);
+  return rc;
+}
+
+extern "C" void hvm_hip(uint32_t* book_buffer) {
+  if (!book_buffer) {
+    fprintf(stderr, "failed to load book\n");
+    return;
+  }
+  hvm_hip_ctx* c = hvm_hip_create(nullptr);
+  if (!c) {
+    fprintf(stderr, "Failed to initialise the GPU runtime: %s\n", g_create_error.c_str());
+    exit(EXIT_FAILURE);
+  }
+  if (!c->load(book_buffer, (size_t)-1)) {
+    fprintf(stderr, "%s\n", c->last_error.c_str());
+    fprintf(stderr, "failed to load book\n");
+    hvm_hip_destroy(c);
+    return;
+  }
+  setlinebuf(stdout);
+  setlinebuf(stderr);
+  hvm_hip_out o;
+  int rc = c->eval(&o);
+  if (rc == HVM_HIP_E_IO) {
+    fprintf(stderr, "IO calls are not supported by the HIP runtime\n");
+  } else if (rc != HVM_HIP_OK) {
+    fprintf(stderr, "Failed to run the evaluator: %s\n", c->last_error.c_str());
+    hvm_hip_destroy(c);
+    exit(EXIT_FAILURE);
+  }
+  printf("Result: %s\n", o.result ? o.result : "");
+  printf("- ITRS: %llu\n", (unsigned long long)o.itrs);
+  printf("- LEAK: %llu\n", (unsigned long long)o.leak);
+  printf("- TIME: %.2fs\n", o.wall_s);
+  printf("- MIPS: %.2f\n", o.wall_s > 0 ? (double)o.itrs / o.wall_s / 1e6 : 0.0);
+  free(o.result);
+  free(o.canonical);
+  hvm_hip_destroy(c);
+}
+
+// ---------------------------------------------------------------------------
+// Front end C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int hvm_front_compile(const char* code, uint32_t** words, size_t* n_words, char** err) {
+  if (words) *words = nullptr;
+  if (n_words) *n_words = 0;
+  if (err) *err = nullptr;
+  if (!code || !words || !n_words) return HVM_HIP_E_ARG;
+  try {
+    hvm::AstBook ast = hvm::parse_book(code);
+    hvm::Book b = hvm::build_book(ast);
+    std::vector<uint32_t> w = hvm::to_buffer(b);
+    uint32_t* p = (uint32_t*)malloc(w.size() * 4 + 4);
+    memcpy(p, w.data(), w.size() * 4);
+    *words = p;
+    *n_words = w.size();
+    return 0;
+  } catch (std::exception& ex) {
+    if (err) *err = dup_cstr(ex.what());
+    return 1;
+  }
+}
+
+extern "C" int hvm_front_canonical(const char* printed, char** out, char** err) {
+  if (out) *out = nullptr;
+  if (err) *err = nullptr;
+  if (!printed || !out) return HVM_HIP_E_ARG;
+  try {
+    *out = dup_cstr(hvm::canonical_tree_text(printed));
+    return 0;
+  } catch (std::exception& ex) {
+    if (err) *err = dup_cstr(ex.what());
+    return 1;
+  }
+}

@@ -1,0 +1,215 @@
+"""Python host for the MI355X HVM2 evaluator (ctypes over lib/libhvm_hip.so).
+
+Mirrors the reference's host interface for the GPU path: `compile(code)` is
+`ast::Book::parse(..).build().to_buffer(..)` (src/main.rs:87-92) and
+`run_hip(words)` is the call into the device runtime (`hvm_cu`, src/main.rs:93-96),
+returning the result text and counters instead of printing them.  The library
+must be built (`make -C lambdavm-rust_amd`); there is no CPU fallback: if the
+shared library or the GPU is missing, these functions raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libhvm_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hvm_hip.h")
+
+KINDS = ["CALL", "CALL_COPY", "VOID", "ERAS", "ANNI", "COMM", "OPER1", "OPER2", "SWIT0", "SWITN"]
+
+# Algorithmic bytes per interaction kind (CALL is per def: 8N + 4V + 8(R+1)).
+KIND_BYTES = {"CALL_COPY": 32, "VOID": 0, "ERAS": 32, "ANNI": 48, "COMM": 112,
+              "OPER1": 32, "OPER2": 40, "SWIT0": 32, "SWITN": 40}
+
+NO_READBACK = 1
+
+STATUS = {0: "ok", 1: "book failed to load", 2: "IO requested (unsupported)",
+          3: "device error", 4: "evaluator error", 5: "bad argument"}
+
+
+class HvmError(RuntimeError):
+    pass
+
+
+class _Cfg(C.Structure):
+    _fields_ = [("node_cap", C.c_uint64), ("vars_cap", C.c_uint64), ("bag_cap", C.c_uint32),
+                ("waves", C.c_uint32), ("turn_steps", C.c_uint32), ("flags", C.c_uint32),
+                ("time_limit_s", C.c_uint32)]
+
+
+class _Out(C.Structure):
+    _fields_ = [("itrs", C.c_uint64), ("kind", C.c_uint64 * 10), ("bytes", C.c_uint64),
+                ("leak", C.c_uint64), ("steps", C.c_uint64), ("spills", C.c_uint64),
+                ("defers", C.c_uint64), ("retries", C.c_uint64), ("probes", C.c_uint64),
+                ("kernel_ms", C.c_double), ("eval_ms", C.c_double), ("wall_s", C.c_double),
+                ("turns", C.c_uint32), ("normalizes", C.c_uint32), ("err", C.c_uint32),
+                ("result", C.c_void_p), ("canonical", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads lib/libhvm_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HvmError(f"{LIB_PATH} is missing: build it with `make -C lambdavm-rust_amd`")
+    L = C.CDLL(LIB_PATH)
+    L.hvm_front_compile.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_uint32)),
+                                    C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]
+    L.hvm_front_compile.restype = C.c_int
+    L.hvm_front_canonical.argtypes = [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+    L.hvm_front_canonical.restype = C.c_int
+    L.hvm_hip_free.argtypes = [C.c_void_p]
+    L.hvm_hip_free.restype = None
+    L.hvm_hip_create.argtypes = [C.POINTER(_Cfg)]
+    L.hvm_hip_create.restype = C.c_void_p
+    L.hvm_hip_load.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.c_size_t]
+    L.hvm_hip_load.restype = C.c_int
+    L.hvm_hip_eval.argtypes = [C.c_void_p, C.POINTER(_Out)]
+    L.hvm_hip_eval.restype = C.c_int
+    L.hvm_hip_destroy.argtypes = [C.c_void_p]
+    L.hvm_hip_destroy.restype = None
+    L.hvm_hip_last_error.argtypes = [C.c_void_p]
+    L.hvm_hip_last_error.restype = C.c_char_p
+    L.hvm_hip_run.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(_Cfg), C.POINTER(_Out)]
+    L.hvm_hip_run.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _take_str(p) -> Optional[str]:
+    if not p:
+        return None
+    s = C.string_at(p).decode("utf-8", "replace")
+    lib().hvm_hip_free(p)
+    return s
+
+
+def compile_book(code: str) -> List[int]:
+    """`.hvm` text → Book::to_buffer words (list of u32)."""
+    L = lib()
+    words = C.POINTER(C.c_uint32)()
+    n = C.c_size_t()
+    err = C.c_void_p()
+    rc = L.hvm_front_compile(code.encode(), C.byref(words), C.byref(n), C.byref(err))
+    if rc != 0:
+        raise HvmError(_take_str(err.value) or "compile failed")
+    out = words[: n.value]
+    L.hvm_hip_free(C.cast(words, C.c_void_p))
+    return list(out)
+
+
+def canonical(printed: str) -> str:
+    """Canonical form of a printed tree (vars renamed x0, x1, ... in order)."""
+    L = lib()
+    out = C.c_void_p()
+    err = C.c_void_p()
+    if L.hvm_front_canonical(printed.encode(), C.byref(out), C.byref(err)) != 0:
+        raise HvmError(_take_str(err.value) or "canonical failed")
+    return _take_str(out.value)
+
+
+def words_array(words) -> C.Array:
+    return (C.c_uint32 * len(words))(*words)
+
+
+@dataclass
+class RunResult:
+    status: int
+    itrs: int = 0
+    kinds: dict = field(default_factory=dict)
+    bytes: int = 0
+    leak: int = 0
+    kernel_ms: float = 0.0
+    eval_ms: float = 0.0
+    wall_s: float = 0.0
+    turns: int = 0
+    normalizes: int = 0
+    err: int = 0
+    result: Optional[str] = None
+    canonical: Optional[str] = None
+    diag: dict = field(default_factory=dict)
+
+
+def _cfg(node_cap=0, vars_cap=0, bag_cap=0, waves=0, turn_steps=0, flags=0, time_limit_s=0) -> _Cfg:
+    return _Cfg(node_cap, vars_cap, bag_cap, waves, turn_steps, flags, time_limit_s)
+
+
+def _result(rc: int, o: _Out) -> RunResult:
+    r = RunResult(status=rc, itrs=o.itrs, kinds={k: o.kind[i] for i, k in enumerate(KINDS)},
+                  bytes=o.bytes, leak=o.leak, kernel_ms=o.kernel_ms, eval_ms=o.eval_ms,
+                  wall_s=o.wall_s, turns=o.turns, normalizes=o.normalizes, err=o.err,
+                  diag={"steps": o.steps, "spills": o.spills, "defers": o.defers,
+                        "retries": o.retries, "probes": o.probes})
+    r.result = _take_str(o.result)
+    r.canonical = _take_str(o.canonical)
+    return r
+
+
+class Context:
+    """A persistent evaluator: net, bags and book stay in HBM between runs."""
+
+    def __init__(self, **cfg):
+        L = lib()
+        self._cfg = _cfg(**cfg)
+        self._h = L.hvm_hip_create(C.byref(self._cfg))
+        if not self._h:
+            raise HvmError("hvm_hip_create failed: " + L.hvm_hip_last_error(None).decode())
+        self._words = None
+
+    def load(self, words) -> None:
+        self._words = words_array(words)
+        rc = lib().hvm_hip_load(self._h, self._words, len(words))
+        if rc != 0:
+            raise HvmError(f"load failed ({STATUS.get(rc, rc)}): " + self.last_error())
+
+    def eval(self) -> RunResult:
+        o = _Out()
+        rc = lib().hvm_hip_eval(self._h, C.byref(o))
+        r = _result(rc, o)
+        if rc != 0:
+            raise HvmError(f"eval failed ({STATUS.get(rc, rc)}, err=0x{r.err:x}): " + self.last_error())
+        return r
+
+    def last_error(self) -> str:
+        return lib().hvm_hip_last_error(self._h).decode()
+
+    def close(self) -> None:
+        if self._h:
+            lib().hvm_hip_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_hip(words, **cfg) -> RunResult:
+    """One-shot evaluation of a book buffer on the GPU (hvm_hip_run)."""
+    L = lib()
+    arr = words_array(words)
+    c = _cfg(**cfg)
+    o = _Out()
+    rc = L.hvm_hip_run(arr, len(words), C.byref(c), C.byref(o))
+    r = _result(rc, o)
+    if rc != 0:
+        raise HvmError(f"hvm_hip_run failed: {STATUS.get(rc, rc)} (err=0x{r.err:x})")
+    return r
+
+
+def run_code(code: str, **cfg) -> RunResult:
+    return run_hip(compile_book(code), **cfg)

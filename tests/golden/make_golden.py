@@ -1,0 +1,105 @@
+"""Writes tests/golden/golden.json: expected normal forms and interaction counts.
+
+Run in the build container (needs /root/reference and oracle/_ref built):
+    python tests/golden/make_golden.py
+
+Each case records the program text (the reference's own test programs and
+examples are its test data; the synthetic workloads come from
+lambdavm-rust_amd/programs.py), the canonical normal form and ITRS printed by
+the compiled reference C runtime (`run-c`, oracle/_ref/hvm_ref_t8), and, for
+the reference programs, the normal form held in the reference's insta
+snapshot (tests/snapshots/*.snap).  Large normal forms are stored as a SHA-256
+of their canonical text.
+"""
+from __future__ import annotations
+
+import glob
+import hashlib
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "lambdavm-rust_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import hvm_hip  # noqa: E402
+import oracle  # noqa: E402
+import programs as P  # noqa: E402
+
+REF = "/root/reference"
+MAX_INLINE = 4096
+
+
+def snapshot_nf(rel: str):
+    """Result line of the reference's snapshot for a program, if any."""
+    key = rel.replace("tests/programs/", "").replace("examples/", "").replace("/", "__")
+    cands = glob.glob(os.path.join(REF, "tests", "snapshots", f"run__file@{key}.snap"))
+    if not cands:
+        return None
+    text = open(cands[0]).read().split("---", 2)[-1].strip()
+    for line in text.splitlines():
+        if line.startswith("Result: "):
+            return line[len("Result: "):]
+    return text
+
+
+def record(name, code, extra=None, threads=8, timeout=3000):
+    words = hvm_hip.compile_book(code)
+    r = oracle.run_reference(words, threads, timeout=timeout)
+    if r.itrs is None:
+        raise RuntimeError(f"{name}: reference produced no ITRS:\n{r.stdout[-500:]}\n{r.stderr[-500:]}")
+    canon = hvm_hip.canonical(r.result)
+    case = {"name": name, "itrs": r.itrs, "ref_time_s": r.time_s,
+            "nf_sha256": hashlib.sha256(canon.encode()).hexdigest(), "nf_len": len(canon)}
+    if len(canon) <= MAX_INLINE:
+        case["nf"] = canon
+    if extra:
+        case.update(extra)
+    print(f"{name}: itrs={r.itrs} t={r.time_s}s nf_len={len(canon)}", flush=True)
+    return case
+
+
+def main():
+    out = {"reference": "compiled reference run-c (src/run.c + src/hvm.c, -O3 -DIO -DTPC_L2=3)",
+           "programs": [], "workloads": []}
+    files = sorted(glob.glob(os.path.join(REF, "examples", "*", "*.hvm")) +
+                   glob.glob(os.path.join(REF, "tests", "programs", "**", "*.hvm"), recursive=True))
+    for path in files:
+        rel = os.path.relpath(path, REF)
+        code = open(path).read()
+        if "@test-io = 1" in code or "@test-skip = 1" in code or "/io/" in rel or "demo_io" in rel:
+            continue
+        try:
+            hvm_hip.compile_book(code)
+        except hvm_hip.HvmError as e:
+            out["programs"].append({"name": rel, "code": code, "compile_error": str(e),
+                                    "snapshot": snapshot_nf(rel)})
+            print(f"{rel}: compile error {e}")
+            continue
+        snap = snapshot_nf(rel)
+        case = record(rel, code, {"code": code, "snapshot": snap})
+        out["programs"].append(case)
+
+    small = [("sum_tree", {"depth": d}) for d in (1, 4, 10, 16)]
+    small += [("bitonic", {"k": k, "seed": s}) for k, s in ((1, 0), (3, 1), (6, 2), (8, 3), (10, 4), (12, 5))]
+    small += [("radix", {"k": k, "seed": s}) for k, s in ((1, 0), (4, 1), (8, 2), (10, 3), (12, 4))]
+    small += [("church", {"n": n, "m": m}) for n, m in ((1, 2), (3, 4), (6, 8), (8, 12))]
+    full = [("sum_tree", {"depth": 20}), ("bitonic", {"k": 16, "seed": 0}),
+            ("radix", {"k": 18, "seed": 0}), ("church", {"n": 12, "m": 12})]
+    for gen, params in small + full:
+        code, expect = getattr(P, gen)(**params)
+        name = gen + "_" + "_".join(f"{k}{v}" for k, v in params.items())
+        case = record(name, code, {"generator": gen, "params": params,
+                                   "full_size": (gen, params) in full})
+        if gen == "bitonic":
+            case["closed_form"] = "sorted keys"
+        else:
+            case["closed_form"] = str(expect)
+        out["workloads"].append(case)
+    with open(os.path.join(ROOT, "tests", "golden", "golden.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
