@@ -1,0 +1,135 @@
+"""Benchmark: MIPS of the HIP evaluator on one MI355X.
+
+Workload (one step): full normalisation of a bitonic sort of 2^16 u24 keys
+drawn from SplitMix64(seed 0), stored as a perfect depth-16 tree — the program
+bitonic(16, 0) of lambdavm-rust_amd/programs.py.  The book, net and bags are
+resident in HBM before timing; each step clears the net, boots @main and runs
+the evaluator to normal form (with the IO driver's readback expansions).
+Steps are timed with the wall clock around K steps; the dominant kernel
+(hvm_turn_kernel) is timed with HIP events on its own stream inside the
+library.  Parity (ITRS and normal form vs the reference run-c golden record)
+is checked on a warmup step outside the timed region.
+
+Prints one JSON line (see the contract in the repository instructions).
+"""
+import argparse
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lambdavm-rust_amd"))
+
+METRIC = "MIPS (interactions/sec) learned vs plain sched, 1 GPU, bit-exact NF; %HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(words, itrs_expected):
+    """Reference run-c (oracle/_ref) on this host's cores, else the C port."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    ncpu = min(ncpu, 16)  # the GPU box's CPU share
+    threads = 1
+    while threads * 2 <= ncpu and threads * 2 <= 32:
+        threads *= 2
+    if oracle.have_reference(threads):
+        r = oracle.run_reference(words, threads, timeout=300)
+        if r.itrs == itrs_expected and r.time_s:
+            return {"value": round(r.itrs / r.time_s / 1e6, 2), "unit": "MIPS", "cores": threads,
+                    "kind": "reference",
+                    "sample": f"one full run of the bench workload by the reference run-c "
+                              f"(src/run.c, -O3 -DIO, TPC={threads}); time window boot→result as run-c prints it"}
+    o = oracle.run_oracle(words)
+    return {"value": round(o.itrs / o.seconds / 1e6, 2), "unit": "MIPS", "cores": 1, "kind": "port",
+            "sample": "one full run of the bench workload by the single-thread C restatement (oracle/)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--k", type=int, default=16, help="log2 of the key count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(this evaluator runs a net on a single device)", file=sys.stderr)
+        sys.exit(2)
+
+    import hvm_hip
+    import programs as P
+
+    code, sorted_keys = P.bitonic(args.k, 0)
+    words = hvm_hip.compile_book(code)
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+    gcase = next((c for c in golden["workloads"]
+                  if c["generator"] == "bitonic" and c["params"] == {"k": args.k, "seed": 0}), None)
+
+    ctx = hvm_hip.Context(time_limit_s=600)
+    ctx.load(words)
+    # parity step (untimed): full readback, compared with the golden record
+    r0 = ctx.eval()
+    parity = {"itrs_match": gcase is not None and r0.itrs == gcase["itrs"],
+              "nf_match": gcase is not None and
+              hashlib.sha256(r0.canonical.encode()).hexdigest() == gcase["nf_sha256"],
+              "sorted_keys_match": P.flatten_tree(P.parse_tree_text(r0.result)) == sorted_keys}
+    ctx.close()
+
+    ctx = hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK)
+    ctx.load(words)
+    for _ in range(max(0, args.warmup - 1)):
+        ctx.eval()
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        runs.append(ctx.eval())
+    t1 = time.perf_counter()
+    ctx.close()
+
+    itrs = runs[0].itrs
+    assert all(r.itrs == itrs for r in runs), "ITRS changed between steps"
+    total_s = t1 - t0
+    ms_per_step = total_s / args.steps * 1e3
+    value = itrs * args.steps / total_s / 1e6
+    kernel_ms = sum(r.kernel_ms for r in runs)
+    turns = sum(r.turns for r in runs)
+    bytes_total = sum(r.bytes for r in runs)
+    avg_launch_ms = kernel_ms / turns
+    bytes_per_launch = bytes_total / turns
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "MIPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"bitonic sort of 2^{args.k} u24 keys (SplitMix64 seed 0), depth-{args.k} key tree",
+                   "keys": 1 << args.k, "itrs_per_step": itrs, "scheduler": "plain"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "hvm_turn_kernel", "launches_per_step": turns / args.steps,
+                     "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
+                     "avg_launch_ms": round(avg_launch_ms, 4)},
+        "kernel_mips": round(itrs * args.steps / (kernel_ms * 1e-3) / 1e6, 2),
+        "eval_ms_per_step": round(sum(r.eval_ms for r in runs) / args.steps, 3),
+        "parity": parity,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(words, itrs)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

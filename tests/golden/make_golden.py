@@ -81,6 +81,26 @@ def main():
         case = record(rel, code, {"code": code, "snapshot": snap})
         out["programs"].append(case)
 
+    # Edge cases: results that are an eraser, a number, a REF (expanded by the
+    # IO driver's readback), free variables, the safe-REF copy rule, erasing a
+    # large structure, and a long list.
+    edges = {
+        "edge_era": "@main = *",
+        "edge_num": "@main = 42",
+        "edge_ref_result": "@main = @f\n@f = (a a)",
+        "edge_ref_number": "@main = @g\n@g = r & 1 ~ $([+] $(2 r))",
+        "edge_free_vars": "@main = (a (b (b a)))",
+        "edge_safe_ref_copy": "@main = (a b) & @id ~ {a b}\n@id = (x x)",
+        "edge_unsafe_ref_expand": "@main = (a b) & @d ~ {a b}\n@d = ({x y} (x y))",
+        "edge_erase_tree": "@main = r & @mk ~ (12 *) & 7 ~ r\n"
+                           "@mk = (?((0 @mk_s) r) r)\n@mk_s = ({a b} (l u)) & @mk ~ (a l) & @mk ~ (b u)",
+        "edge_void_pairs": "@main = r & * ~ 1 & @f ~ * & 3 ~ @f & r ~ 5\n@f = *",
+        "edge_list_3000": "@main = r & @ls ~ (3000 r)\n@ls = (?((* @ls_s) r) r)\n"
+                          "@ls_s = ({a b} (a t)) & @ls ~ (b t)",
+    }
+    for name, code in edges.items():
+        out["programs"].append(record(name, code, {"code": code, "snapshot": None}))
+
     small = [("sum_tree", {"depth": d}) for d in (1, 4, 10, 16)]
     small += [("bitonic", {"k": k, "seed": s}) for k, s in ((1, 0), (3, 1), (6, 2), (8, 3), (10, 4), (12, 5))]
     small += [("radix", {"k": k, "seed": s}) for k, s in ((1, 0), (4, 1), (8, 2), (10, 3), (12, 4))]
