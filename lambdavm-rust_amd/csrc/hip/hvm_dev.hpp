@@ -64,6 +64,21 @@ struct HvmWaveState {
   uint32_t vars_cur;
 };
 
+// Turn control block, in device memory.  The turn kernel reads it at start;
+// the between kernel advances it, so a chain of turns needs no host round trip.
+struct HvmCtl {
+  uint32_t n_in;        // redexes in the input bag of the current turn
+  uint32_t n_out;       // output-bag counter
+  uint32_t in_head;     // dynamic-pull counter into the input bag
+  uint32_t err;         // HVM_ERR_* flags
+  uint32_t parity;      // input bag index (0/1)
+  uint32_t max_steps;   // steps of the current turn
+  uint32_t turns;       // turns completed
+  uint32_t done;        // set once a turn leaves no redex (or on error)
+  uint32_t fixed_steps; // if nonzero, every turn runs this many steps
+  uint32_t pad[7];
+};
+
 struct HvmTurnArgs {
   // net
   unsigned long long* node;
@@ -74,19 +89,14 @@ struct HvmTurnArgs {
   const HvmDevDef* defs;
   const unsigned long long* pairs;
   uint32_t ndefs;
-  // bags
-  const unsigned long long* bag_in;
-  uint32_t n_in;
-  unsigned long long* bag_out;
-  uint32_t* n_out;                 // device counter
-  uint32_t* in_head;               // device counter for dynamic pulls
+  // bags: bag[ctl->parity] is the input, the other one the output
+  unsigned long long* bag[2];
   uint32_t bag_cap;
   // control
-  uint32_t max_steps;              // steps per turn
+  HvmCtl* ctl;
   uint32_t nwaves;
   HvmWaveState* wave_state;
   HvmDevStats* stats;
-  uint32_t* err;
 };
 
 struct HvmGatherArgs {

@@ -39,21 +39,21 @@ __device__ __forceinline__ u32 vidx(u32 val) { return val == HVM_ROOT_VAL ? 0u :
 // points outside the net raises HVM_ERR_BAD_PORT instead of touching memory
 // the net does not own.
 __device__ __forceinline__ u64 node_take(const HvmTurnArgs& A, u32 loc) {
-  if (loc >= A.node_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return 0; }
+  if (loc >= A.node_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return 0; }
   return __hip_atomic_exchange(&A.node[loc], 0ull, __ATOMIC_RELAXED, AGENT);
 }
 __device__ __forceinline__ void node_put(const HvmTurnArgs& A, u32 loc, u64 v) {
-  if (loc >= A.node_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return; }
+  if (loc >= A.node_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return; }
   __hip_atomic_store(&A.node[loc], v, __ATOMIC_RELAXED, AGENT);
 }
 __device__ __forceinline__ u32 vars_xchg(const HvmTurnArgs& A, u32 val, u32 x) {
   u32 i = vidx(val);
-  if (i >= A.vars_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return HVM_NONE; }
+  if (i >= A.vars_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return HVM_NONE; }
   return __hip_atomic_exchange(&A.vars[i], x, __ATOMIC_RELAXED, AGENT);
 }
 __device__ __forceinline__ void vars_put(const HvmTurnArgs& A, u32 val, u32 x) {
   u32 i = vidx(val);
-  if (i >= A.vars_cap) { atomicOr(A.err, (u32)HVM_ERR_BAD_PORT); return; }
+  if (i >= A.vars_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return; }
   __hip_atomic_store(&A.vars[i], x, __ATOMIC_RELAXED, AGENT);
 }
 
@@ -125,9 +125,9 @@ __device__ __forceinline__ void push_redex(WaveLds& L, Lane& ln, const HvmTurnAr
   if (ln.top - ln.bot == HVM_DEQ) {
     u64 old = deq_at(L, ln.bot);
     ln.bot++;
-    u32 pos = atomicAdd(A.n_out, 1u);
-    if (pos < A.bag_cap) A.bag_out[pos] = old;
-    else atomicOr(A.err, (u32)HVM_ERR_BAG_OVERFLOW);
+    u32 pos = atomicAdd(&A.ctl->n_out, 1u);
+    if (pos < A.bag_cap) A.bag[A.ctl->parity ^ 1u][pos] = old;
+    else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     ln.spills++;
   }
   deq_at(L, ln.top) = r;
@@ -148,7 +148,7 @@ __device__ __forceinline__ u32 enter(const HvmTurnArgs& A, u32 var) {
     vars_put(A, hvm_get_val(var), 0);
     var = val;
   }
-  atomicOr(A.err, (u32)HVM_ERR_LINK_LOOP);
+  atomicOr(&A.ctl->err, (u32)HVM_ERR_LINK_LOOP);
   return var;
 }
 
@@ -162,7 +162,7 @@ __device__ __forceinline__ void link(WaveLds& L, Lane& ln, const HvmTurnArgs& A,
     vars_put(A, hvm_get_val(a), 0);
     a = a_;
   }
-  atomicOr(A.err, (u32)HVM_ERR_LINK_LOOP);
+  atomicOr(&A.ctl->err, (u32)HVM_ERR_LINK_LOOP);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,6 +216,15 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
   const u32 lane = lane_id();
   const u32 w = blockIdx.x;
 
+  // Turn parameters come from the control block (written by the host before
+  // the first turn and by hvm_between_kernel after every turn).
+  const HvmCtl* C = A.ctl;
+  if (C->done) return;
+  const u32 n_in = C->n_in;
+  const u32 max_steps = C->max_steps;
+  const u64* bag_in = A.bag[C->parity];
+  u64* bag_out = A.bag[C->parity ^ 1u];
+
   Lane ln;
   ln.bot = ln.top = 0;
 #pragma unroll
@@ -234,11 +243,11 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
   // spread one redex per wave before any wave takes a second one.
   {
     u32 idx = w + lane * A.nwaves;
-    if (idx < A.n_in && idx < HVM_WAVE * A.nwaves) push_redex(L, ln, A, A.bag_in[idx]);
+    if (idx < n_in && idx < HVM_WAVE * A.nwaves) push_redex(L, ln, A, bag_in[idx]);
   }
-  bool more = A.n_in > HVM_WAVE * A.nwaves;
+  bool more = n_in > HVM_WAVE * A.nwaves;
 
-  for (u32 step = 0; step < A.max_steps; ++step) {
+  for (u32 step = 0; step < max_steps; ++step) {
     // ---- refill from the input bag ----
     if (more) {
       bool empty = ln.top == ln.bot;
@@ -246,11 +255,11 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       if (em) {
         u32 k = __popcll(em);
         u32 base = 0;
-        if (lane == 0) base = atomicAdd(A.in_head, k);
+        if (lane == 0) base = atomicAdd(&A.ctl->in_head, k);
         base = __shfl(base, 0, HVM_WAVE);
-        if (base >= A.n_in) more = false;
+        if (base >= n_in) more = false;
         u32 r = rank_in(em);
-        if (empty && base + r < A.n_in) push_redex(L, ln, A, A.bag_in[base + r]);
+        if (empty && base + r < n_in) push_redex(L, ln, A, bag_in[base + r]);
       }
     }
 
@@ -279,7 +288,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       if (!more) break;
       continue;
     }
-    if ((step & 63) == 0 && __hip_atomic_load(A.err, __ATOMIC_RELAXED, AGENT) != 0) break;
+    if ((step & 63) == 0 && __hip_atomic_load(&A.ctl->err, __ATOMIC_RELAXED, AGENT) != 0) break;
     steps++;
 
     // ---- P0: peek + classify ----
@@ -316,14 +325,14 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     // ---- P2: allocation ----
     if (need_n) {
       if (!wave_alloc(L, A.node, n_lo, A.node_slice, n_cur, need_n, L.tbl_n, probes)) {
-        atomicOr(A.err, (u32)HVM_ERR_OOM_NODE);
+        atomicOr(&A.ctl->err, (u32)HVM_ERR_OOM_NODE);
         if (run) ln.top++;
         break;
       }
     }
     if (need_v) {
       if (!wave_alloc(L, A.vars, v_lo, A.vars_slice, v_cur, need_v, L.tbl_v, probes)) {
-        atomicOr(A.err, (u32)HVM_ERR_OOM_VARS);
+        atomicOr(&A.ctl->err, (u32)HVM_ERR_OOM_VARS);
         if (run) ln.top++;
         break;
       }
@@ -474,13 +483,30 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     u32 tot;
     u32 off = wave_scan(n, &tot);
     u32 base = 0;
-    if (lane == 0 && tot) base = atomicAdd(A.n_out, tot);
+    if (lane == 0 && tot) base = atomicAdd(&A.ctl->n_out, tot);
     base = __shfl(base, 0, HVM_WAVE);
     if (base + tot > A.bag_cap) {
-      if (lane == 0) atomicOr(A.err, (u32)HVM_ERR_BAG_OVERFLOW);
+      if (lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     } else {
-      for (u32 i = 0; i < n; ++i) A.bag_out[base + off + i] = deq_at(L, ln.bot + i);
+      for (u32 i = 0; i < n; ++i) bag_out[base + off + i] = deq_at(L, ln.bot + i);
     }
+  }
+
+  // ---- input redexes no wave pulled this turn are carried to the output ----
+  while (more) {
+    u32 base = 0;
+    if (lane == 0) base = atomicAdd(&A.ctl->in_head, (u32)HVM_WAVE);
+    base = __shfl(base, 0, HVM_WAVE);
+    if (base >= n_in) break;
+    u32 cnt = min((u32)HVM_WAVE, n_in - base);
+    u32 obase = 0;
+    if (lane == 0) obase = atomicAdd(&A.ctl->n_out, cnt);
+    obase = __shfl(obase, 0, HVM_WAVE);
+    if (obase + cnt > A.bag_cap) {
+      if (lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
+      break;
+    }
+    if (lane < cnt) bag_out[obase + lane] = bag_in[base + lane];
   }
 
   // ---- statistics: one atomic per counter per wave ----
@@ -561,12 +587,54 @@ __global__ __launch_bounds__(256) void hvm_gather_kernel(HvmGatherArgs G) {
 }
 
 // ---------------------------------------------------------------------------
+// Between turns: the output bag becomes the next input, and the next turn's
+// length is chosen from the number of pending redexes.  One thread.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ u32 turn_policy(u32 n, u32 nwaves) {
+  u32 lanes = nwaves * HVM_WAVE;
+  if (n < nwaves) return 24;        // spreading: short turns multiply the redexes
+  if (n < lanes / 4) return 96;     // growing
+  return 4096;                      // every lane has work: long turns
+}
+
+__global__ void hvm_between_kernel(HvmTurnArgs A) {
+  HvmCtl* c = A.ctl;
+  if (threadIdx.x != 0 || c->done) return;
+  u32 n = c->n_out;
+  c->turns += 1;
+  if (c->err || n > A.bag_cap) {
+    if (n > A.bag_cap) c->err |= HVM_ERR_BAG_OVERFLOW;
+    c->done = 1;
+    return;
+  }
+  c->n_in = n;
+  c->n_out = 0;
+  c->in_head = min(n, A.nwaves * (u32)HVM_WAVE);
+  c->parity ^= 1u;
+  c->max_steps = c->fixed_steps ? c->fixed_steps : turn_policy(n, A.nwaves);
+  if (n == 0) c->done = 1;
+}
+
+// ---------------------------------------------------------------------------
 // Launch helpers (host)
 // ---------------------------------------------------------------------------
 
 extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s) {
   hipLaunchKernelGGL(hvm_turn_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
   return hipGetLastError();
+}
+
+extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, hipStream_t s) {
+  hipLaunchKernelGGL(hvm_between_kernel, dim3(1), dim3(HVM_WAVE), 0, s, *a);
+  return hipGetLastError();
+}
+
+extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves) {
+  uint32_t lanes = nwaves * HVM_WAVE;
+  if (n < nwaves) return 24;
+  if (n < lanes / 4) return 96;
+  return 4096;
 }
 
 extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s) {

@@ -17,6 +17,8 @@
 
 extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s);
 extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s);
+extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, hipStream_t s);
+extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves);
 extern "C" size_t hvm_turn_lds_bytes(void);
 
 namespace {
@@ -51,7 +53,7 @@ struct hvm_hip_ctx {
   // bags + control
   u64* bag[2] = {nullptr, nullptr};
   u32 bag_cap = 0;
-  u32* ctl = nullptr;          // [0] n_out, [1] in_head, [2] err
+  HvmCtl* ctl = nullptr;       // turn control block
   HvmWaveState* wave_state = nullptr;
   HvmDevStats* stats = nullptr;
   // book
@@ -104,7 +106,7 @@ struct hvm_hip_ctx {
     if (!ck(hipMalloc(&vars, (size_t)vars_cap * 4), "hipMalloc(vars)")) return false;
     if (!ck(hipMalloc(&bag[0], (size_t)bag_cap * 8), "hipMalloc(bag)")) return false;
     if (!ck(hipMalloc(&bag[1], (size_t)bag_cap * 8), "hipMalloc(bag)")) return false;
-    if (!ck(hipMalloc(&ctl, 16 * sizeof(u32)), "hipMalloc(ctl)")) return false;
+    if (!ck(hipMalloc(&ctl, sizeof(HvmCtl)), "hipMalloc(ctl)")) return false;
     if (!ck(hipMalloc(&wave_state, nwaves * sizeof(HvmWaveState)), "hipMalloc(wave_state)")) return false;
     if (!ck(hipMalloc(&stats, sizeof(HvmDevStats)), "hipMalloc(stats)")) return false;
     return true;
@@ -122,6 +124,7 @@ struct hvm_hip_ctx {
     if (d_pairs) hipFree(d_pairs);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
+    for (hipEvent_t e : tev) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
   }
 
@@ -208,7 +211,7 @@ struct hvm_hip_ctx {
     if (!ck(hipMemsetAsync(vars, 0, (size_t)vars_cap * 4, stream), "clear vars")) return false;
     if (!ck(hipMemsetAsync(wave_state, 0, nwaves * sizeof(HvmWaveState), stream), "clear cursors")) return false;
     if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return false;
-    if (!ck(hipMemsetAsync(ctl, 0, 16 * sizeof(u32), stream), "clear ctl")) return false;
+    if (!ck(hipMemsetAsync(ctl, 0, sizeof(HvmCtl), stream), "clear ctl")) return false;
     return ck(hipStreamSynchronize(stream), "reset sync");
   }
 
@@ -217,67 +220,78 @@ struct hvm_hip_ctx {
     u32 none = HVM_NONE;
     if (!ck(hipMemcpyAsync(vars, &none, 4, hipMemcpyHostToDevice, stream), "boot root")) return false;
     if (!ck(hipMemcpyAsync(bag[0], &redex, 8, hipMemcpyHostToDevice, stream), "boot redex")) return false;
-    pending = 1;
-    cur = 0;
+    HvmCtl c;
+    memset(&c, 0, sizeof c);
+    c.n_in = 1;
+    c.in_head = 1;
+    c.parity = 0;
+    c.fixed_steps = cfg.turn_steps;
+    c.max_steps = cfg.turn_steps ? cfg.turn_steps : hvm_turn_policy_host(1, nwaves);
+    c.turns = 0;
+    c.done = 0;
+    host_ctl = c;
+    if (!ck(hipMemcpyAsync(ctl, &host_ctl, sizeof(HvmCtl), hipMemcpyHostToDevice, stream), "boot ctl")) return false;
     return ck(hipStreamSynchronize(stream), "boot sync");
   }
 
-  u32 pending = 0;  // redexes in bag[cur]
-  int cur = 0;
+  HvmCtl host_ctl{};
+  std::vector<hipEvent_t> tev;  // a start/stop event pair per turn of a chunk
 
-  u32 turn_steps(u32 n) const {
-    if (cfg.turn_steps) return cfg.turn_steps;
-    u64 lanes = (u64)nwaves * HVM_WAVE;
-    if (n < nwaves) return 24;
-    if (n < lanes / 4) return 96;
-    return 4096;
+  HvmTurnArgs turn_args() const {
+    HvmTurnArgs a;
+    a.node = node; a.vars = vars;
+    a.node_cap = node_cap; a.vars_cap = vars_cap;
+    a.node_slice = node_slice; a.vars_slice = vars_slice;
+    a.defs = d_defs; a.pairs = d_pairs; a.ndefs = ndefs;
+    a.bag[0] = bag[0]; a.bag[1] = bag[1];
+    a.bag_cap = bag_cap;
+    a.ctl = ctl;
+    a.nwaves = nwaves;
+    a.wave_state = wave_state;
+    a.stats = stats;
+    return a;
   }
 
-  // The turn loop: replaces gnet_normalize (src/hvm.cu:1810-1839).
   double deadline = 0;  // steady-clock seconds; 0 = no limit
 
+  // The turn loop (replaces gnet_normalize, src/hvm.cu:1810-1839).  Turns are
+  // launched in chunks of (turn, between) kernel pairs with no host round trip
+  // inside a chunk; the host reads the control block once per chunk.  Turns
+  // launched after the normal form is reached return immediately.
   bool normalize() {
     normalizes++;
-    while (pending > 0) {
-      if (deadline > 0 && now_s() > deadline) return fail("time limit exceeded");
-      HvmTurnArgs a;
-      a.node = node; a.vars = vars;
-      a.node_cap = node_cap; a.vars_cap = vars_cap;
-      a.node_slice = node_slice; a.vars_slice = vars_slice;
-      a.defs = d_defs; a.pairs = d_pairs; a.ndefs = ndefs;
-      a.bag_in = bag[cur]; a.n_in = pending;
-      a.bag_out = bag[cur ^ 1];
-      a.n_out = ctl + 0; a.in_head = ctl + 1; a.err = ctl + 2;
-      a.bag_cap = bag_cap;
-      a.max_steps = turn_steps(pending);
-      a.nwaves = nwaves;
-      a.wave_state = wave_state;
-      a.stats = stats;
-      u32 init[2] = {0, (u32)std::min<u64>((u64)nwaves * HVM_WAVE, pending)};
-      if (!ck(hipMemcpyAsync(ctl, init, 8, hipMemcpyHostToDevice, stream), "turn init")) return false;
-      hipEventRecord(ev0, stream);
-      if (!ck(hvm_launch_turn(&a, stream), "launch turn")) return false;
-      hipEventRecord(ev1, stream);
-      u32 back[3];
-      if (!ck(hipMemcpyAsync(back, ctl, 12, hipMemcpyDeviceToHost, stream), "turn readback")) return false;
-      if (!ck(hipStreamSynchronize(stream), "turn sync")) return false;
-      float ms = 0;
-      hipEventElapsedTime(&ms, ev0, ev1);
-      kernel_ms += ms;
-      turns++;
-      if (back[2]) return fail("evaluator error flags 0x" + std::to_string(back[2]));
-      u32 n_out = back[0], head = back[1];
-      if (n_out > bag_cap) return fail("redex bag overflow");
-      if (head < pending) {
-        // Redexes of the input bag that no wave pulled this turn carry over.
-        u32 rem = pending - head;
-        if ((u64)n_out + rem > bag_cap) return fail("redex bag overflow");
-        if (!ck(hipMemcpyAsync(bag[cur ^ 1] + n_out, bag[cur] + head, (size_t)rem * 8,
-                               hipMemcpyDeviceToDevice, stream), "carry")) return false;
-        n_out += rem;
+    HvmTurnArgs a = turn_args();
+    u32 chunk = 2;
+    const u32 max_chunk = 64;
+    if (tev.size() < 2 * max_chunk) {
+      for (size_t i = tev.size(); i < 2 * max_chunk; ++i) {
+        hipEvent_t e;
+        if (!ck(hipEventCreate(&e), "hipEventCreate")) return false;
+        tev.push_back(e);
       }
-      pending = n_out;
-      cur ^= 1;
+    }
+    u32 turns_before = host_ctl.turns;
+    for (;;) {
+      if (deadline > 0 && now_s() > deadline) return fail("time limit exceeded");
+      for (u32 i = 0; i < chunk; ++i) {
+        hipEventRecord(tev[2 * i], stream);
+        if (!ck(hvm_launch_turn(&a, stream), "launch turn")) return false;
+        hipEventRecord(tev[2 * i + 1], stream);
+        if (!ck(hvm_launch_between(&a, stream), "launch between")) return false;
+      }
+      if (!ck(hipMemcpyAsync(&host_ctl, ctl, sizeof(HvmCtl), hipMemcpyDeviceToHost, stream), "ctl readback")) return false;
+      if (!ck(hipStreamSynchronize(stream), "turn sync")) return false;
+      u32 ran = host_ctl.turns - turns_before;
+      for (u32 i = 0; i < chunk && i < ran; ++i) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]);
+        kernel_ms += ms;
+      }
+      turns += ran;
+      turns_before = host_ctl.turns;
+      if (host_ctl.err) return fail("evaluator error flags 0x" + std::to_string(host_ctl.err));
+      if (host_ctl.done) break;
+      if (chunk < max_chunk) chunk *= 2;
     }
     return true;
   }
@@ -453,7 +467,7 @@ struct hvm_hip_ctx {
     }
     if (rc != 0) {
       u32 e = 0;
-      hipMemcpy(&e, ctl + 2, 4, hipMemcpyDeviceToHost);
+      hipMemcpy(&e, &ctl->err, 4, hipMemcpyDeviceToHost);
       if (out) out->err = e;
       return rc;
     }
