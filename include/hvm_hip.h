@@ -102,6 +102,60 @@ void hvm_hip_destroy(hvm_hip_ctx* ctx);
 /* Last error message of a context (or of context creation when ctx is NULL). */
 const char* hvm_hip_last_error(const hvm_hip_ctx* ctx);
 
+/* Batched evaluation of independent nets (no counterpart in the reference,
+ * which evaluates one net per call; the per-net semantics are those of
+ * hvm_hip_eval).  The loaded book holds every program; entries[i] is the fid
+ * of net i's entry def.  Nets are admitted in order while the sum of their
+ * estimated peak node counts (est_nodes, or NULL for "admit all") fits the
+ * net; each admitted group is booted into one heap, each net with its own
+ * root var, and normalised by one chain of turns.  outs[i] receives net i's
+ * result text (itrs of a single net is not separable and is left 0); *total
+ * receives the summed counters.  At most HVM_HIP_MAX_NETS nets per group. */
+#define HVM_HIP_MAX_NETS 4096u
+int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_nets,
+                       const uint64_t* est_nodes, hvm_hip_out* outs, hvm_hip_out* total);
+
+/* ------------------------------------------------------------------------ */
+/* Scheduler (no counterpart in the reference: its CUDA runtime switches    */
+/* between fixed SEED/GROW/WORK modes, src/hvm.cu:1509-1548)                */
+/* ------------------------------------------------------------------------ */
+
+/* After every turn the scheduler picks the next turn's length from
+ * {24, 96, 512, 4096} steps, i.e. how long waves work on their own redexes
+ * before the HBM bag redistributes them.  It never changes what an
+ * interaction does, so normal form and ITRS do not depend on it.
+ *   HVM_HIP_SCHED_PLAIN   fixed heuristic on the number of pending redexes
+ *   HVM_HIP_SCHED_LEARNED argmax of a 32->64->64->4 MLP (fp16 params,
+ *                         HVM_HIP_MLP_PARAMS values) evaluated on the device
+ *   HVM_HIP_SCHED_EXPLORE the heuristic with explore_pct% random choices
+ *                         (seeded), for collecting training traces
+ * trace_cap > 0 records one HVM_HIP_TRACE_REC-float record per decision:
+ * 32 features, action, reward (log2 interactions per µs of the turn the
+ * action governed), interactions, µs. */
+#define HVM_HIP_SCHED_PLAIN   0u
+#define HVM_HIP_SCHED_LEARNED 1u
+#define HVM_HIP_SCHED_EXPLORE 2u
+#define HVM_HIP_MLP_FEAT   32
+#define HVM_HIP_MLP_PARAMS (32 * 64 + 64 + 64 * 64 + 64 + 64 * 4 + 4)
+#define HVM_HIP_TRACE_REC  36
+int hvm_hip_set_scheduler(hvm_hip_ctx* ctx, uint32_t mode, uint32_t seed, uint32_t explore_pct,
+                          const uint16_t* params, uint32_t trace_cap);
+int hvm_hip_read_trace(hvm_hip_ctx* ctx, float* recs, uint32_t cap, uint32_t* n_recs);
+
+/* The scheduler MLP's training kernels (HIP, fp16 weights, fp32 math).
+ * Parameters are packed W1[32][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4].
+ * Loss per sample: 0.5 * (y[action] - reward)^2 (only the taken action).
+ *   hvm_mlp_forward: y[n][4] for inputs x[n][32]
+ *   hvm_mlp_grad:    mean gradient (fp32, packed like params) and mean loss
+ *   hvm_mlp_train:   `epochs` full-batch Adam steps from params_in (fp32
+ *                    master copy), writes fp16 params_out and the final loss */
+int hvm_mlp_forward(const uint16_t* params, const float* x, uint32_t n, float* y);
+int hvm_mlp_grad(const uint16_t* params, const float* x, const uint32_t* actions, const float* rewards,
+                 uint32_t n, float* grad, float* loss);
+int hvm_mlp_train(const float* params_in, const float* x, const uint32_t* actions, const float* rewards,
+                  uint32_t n, uint32_t epochs, float lr, uint16_t* params_out, float* params_fp32_out,
+                  float* loss);
+
 /* ------------------------------------------------------------------------ */
 /* Front end (the reference's Rust host, restated in C++)                   */
 /* ------------------------------------------------------------------------ */

@@ -79,6 +79,47 @@ struct HvmCtl {
   uint32_t pad[7];
 };
 
+// ---------------------------------------------------------------------------
+// Scheduler: picks the length of the next turn (how long waves run on their
+// own redexes before the HBM bag redistributes work).  It never changes what
+// an interaction does, only when redexes are shared.
+// ---------------------------------------------------------------------------
+
+#define HVM_ACTIONS     4      // turn lengths the scheduler chooses from
+#define HVM_FEAT        32     // MLP input features
+#define HVM_HID         64     // MLP hidden width
+// packed fp16 parameters: W1[32][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4]
+#define HVM_MLP_PARAMS  (HVM_FEAT * HVM_HID + HVM_HID + HVM_HID * HVM_HID + HVM_HID + HVM_HID * HVM_ACTIONS + HVM_ACTIONS)
+// trace record: features, action, reward (log2 interactions per µs), itrs, dt µs
+#define HVM_TRACE_REC   (HVM_FEAT + 4)
+
+enum { HVM_SCHED_PLAIN = 0, HVM_SCHED_LEARNED = 1, HVM_SCHED_EXPLORE = 2 };
+
+// Per-turn counters, summed by the turn kernel and consumed (then cleared)
+// by the between kernel.
+struct HvmTurnStats {
+  unsigned long long itrs;
+  unsigned long long wave_steps;   // steps in which a wave had work
+  unsigned long long lane_steps;   // lanes with work, summed over those steps
+  unsigned long long spills;
+  unsigned long long defers;
+  unsigned long long probes;
+  unsigned long long kind[HVM_K_KINDS];
+};
+
+struct HvmSched {
+  uint32_t mode;                 // HVM_SCHED_*
+  uint32_t seed;                 // exploration jitter
+  uint32_t explore_pct;          // % of turns with a random action (explore mode)
+  uint32_t trace_cap;            // records the trace buffer holds
+  uint32_t trace_len;            // records written
+  uint32_t last_action;
+  unsigned long long last_clock; // wall clock (100 MHz) at the previous decision
+  const unsigned short* params;  // HVM_MLP_PARAMS fp16 values (learned mode)
+  float* trace;                  // trace_cap * HVM_TRACE_REC floats, or null
+  HvmTurnStats turn;
+};
+
 struct HvmTurnArgs {
   // net
   unsigned long long* node;
@@ -97,6 +138,8 @@ struct HvmTurnArgs {
   uint32_t nwaves;
   HvmWaveState* wave_state;
   HvmDevStats* stats;
+  uint32_t root_vars;              // var slots [0, root_vars) are roots, never allocated
+  HvmSched* sched;
 };
 
 struct HvmGatherArgs {

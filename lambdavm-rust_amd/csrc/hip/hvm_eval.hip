@@ -20,6 +20,7 @@
 // more (in-wave sharing through LDS).  Redexes that overflow a deque, and all
 // redexes left at the end of a turn, go to the HBM output bag, which the host
 // hands to the next turn; a turn with an empty output bag ends normalisation.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "hvm_dev.hpp"
@@ -175,7 +176,7 @@ __device__ __forceinline__ void link(WaveLds& L, Lane& ln, const HvmTurnArgs& A,
 
 template <typename T>
 __device__ __forceinline__ bool wave_alloc(WaveLds& L, T* buf, u32 lo, u32 len, u32& cur, u32 need,
-                                           u32* tbl, unsigned long long& probes) {
+                                           u32* tbl, unsigned long long& probes, u32 reserved) {
   u32 lane = lane_id();
   u32 got = 0;
   u32 scanned = 0;
@@ -183,7 +184,7 @@ __device__ __forceinline__ bool wave_alloc(WaveLds& L, T* buf, u32 lo, u32 len, 
     u32 rel = (cur - lo + lane) % len;
     u32 idx = lo + rel;
     T v = __hip_atomic_load(&buf[idx], __ATOMIC_RELAXED, AGENT);
-    bool free_ = (v == 0) && (idx != 0);
+    bool free_ = (v == 0) && (idx >= reserved);
     u64 m = __ballot(free_);
     u32 r = rank_in(m);
     u32 nfree = __popcll(m);
@@ -231,9 +232,10 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
   for (int k = 0; k < HVM_K_KINDS; ++k) ln.kind[k] = 0;
   ln.bytes = 0;
   ln.spills = 0;
-  u64 probes = 0, steps = 0, defers = 0, retries = 0;
+  u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0;
 
-  const u32 n_lo = w * A.node_slice, v_lo = w * A.vars_slice;
+  // Slices start after the reserved slots: node slot 0, var slots [0, root_vars).
+  const u32 n_lo = 1 + w * A.node_slice, v_lo = A.root_vars + w * A.vars_slice;
   u32 n_cur = A.wave_state[w].node_cur;
   u32 v_cur = A.wave_state[w].vars_cur;
   if (n_cur < n_lo || n_cur >= n_lo + A.node_slice) n_cur = n_lo;
@@ -290,12 +292,16 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     }
     if ((step & 63) == 0 && __hip_atomic_load(&A.ctl->err, __ATOMIC_RELAXED, AGENT) != 0) break;
     steps++;
+    lane_steps += __popcll(__ballot(has));
 
     // ---- P0: peek + classify ----
     u64 rx = has ? deq_at(L, ln.top - 1) : 0ull;
     u32 a = hvm_get_fst(rx), b = hvm_get_snd(rx);
     u32 rule = hvm_get_rule(a, b);
-    if (hvm_get_tag(a) == HVM_REF && b == HVM_ROOT) rule = HVM_CALL;
+    // A REF against a root var is a boot redex: CALL (src/hvm.c:1088-1090).
+    // Var slots [1, root_vars) are the roots of batched nets.
+    if (hvm_get_tag(a) == HVM_REF && hvm_get_tag(b) == HVM_VAR &&
+        (b == HVM_ROOT || (hvm_get_val(b) != 0 && hvm_get_val(b) < A.root_vars))) rule = HVM_CALL;
     else if (hvm_should_swap(a, b)) { u32 x = a; a = b; b = x; }
     bool copy = false;
     HvmDevDef d;
@@ -324,14 +330,14 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
 
     // ---- P2: allocation ----
     if (need_n) {
-      if (!wave_alloc(L, A.node, n_lo, A.node_slice, n_cur, need_n, L.tbl_n, probes)) {
+      if (!wave_alloc(L, A.node, n_lo, A.node_slice, n_cur, need_n, L.tbl_n, probes, 1u)) {
         atomicOr(&A.ctl->err, (u32)HVM_ERR_OOM_NODE);
         if (run) ln.top++;
         break;
       }
     }
     if (need_v) {
-      if (!wave_alloc(L, A.vars, v_lo, A.vars_slice, v_cur, need_v, L.tbl_v, probes)) {
+      if (!wave_alloc(L, A.vars, v_lo, A.vars_slice, v_cur, need_v, L.tbl_v, probes, A.root_vars)) {
         atomicOr(&A.ctl->err, (u32)HVM_ERR_OOM_VARS);
         if (run) ln.top++;
         break;
@@ -517,6 +523,8 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     u64 s_itrs = wave_sum64(itrs);
     u64 s_bytes = wave_sum64(ln.bytes);
     u64 s_spill = wave_sum64(ln.spills);
+    defers = wave_sum64(defers);    // per-lane counts; steps/probes/lane_steps are wave-uniform
+    retries = wave_sum64(retries);
     u64 kinds[HVM_K_KINDS];
 #pragma unroll
     for (int k = 0; k < HVM_K_KINDS; ++k) kinds[k] = wave_sum64(ln.kind[k]);
@@ -532,6 +540,16 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       if (probes) atomicAdd(&A.stats->probes, probes);
       A.wave_state[w].node_cur = n_cur;
       A.wave_state[w].vars_cur = v_cur;
+      // per-turn counters for the scheduler
+      HvmTurnStats* T = &A.sched->turn;
+      if (s_itrs) atomicAdd(&T->itrs, s_itrs);
+      if (steps) atomicAdd(&T->wave_steps, steps);
+      if (lane_steps) atomicAdd(&T->lane_steps, lane_steps);
+      if (s_spill) atomicAdd(&T->spills, s_spill);
+      if (defers) atomicAdd(&T->defers, defers);
+      if (probes) atomicAdd(&T->probes, probes);
+#pragma unroll
+      for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&T->kind[k], kinds[k]);
     }
   }
 }
@@ -587,33 +605,135 @@ __global__ __launch_bounds__(256) void hvm_gather_kernel(HvmGatherArgs G) {
 }
 
 // ---------------------------------------------------------------------------
-// Between turns: the output bag becomes the next input, and the next turn's
-// length is chosen from the number of pending redexes.  One thread.
+// Between turns: the output bag becomes the next input, and the scheduler
+// chooses the next turn's length.  One wave; thread 0 does the bookkeeping,
+// all 64 lanes evaluate the MLP in learned mode.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ u32 turn_policy(u32 n, u32 nwaves) {
+__device__ const u32 kTurnLen[HVM_ACTIONS] = {24, 96, 512, 4096};
+
+// The fixed heuristic: short turns while spreading a few redexes over the
+// waves, medium while growing, long once every lane can have work.
+__host__ __device__ __forceinline__ u32 plain_action(u32 n, u32 nwaves) {
   u32 lanes = nwaves * HVM_WAVE;
-  if (n < nwaves) return 24;        // spreading: short turns multiply the redexes
-  if (n < lanes / 4) return 96;     // growing
-  return 4096;                      // every lane has work: long turns
+  if (n < nwaves) return 0;
+  if (n < lanes / 4) return 1;
+  return 3;
 }
 
-__global__ void hvm_between_kernel(HvmTurnArgs A) {
+__device__ __forceinline__ float log2p(float x) { return __log2f(1.0f + x); }
+
+// Scheduler features of the turn that just ended (all in [0, ~1]).
+__device__ void sched_features(const HvmCtl* c, const HvmSched* s, u32 n_next, u32 nwaves, float dt_us,
+                               float* f) {
+  const HvmTurnStats& T = s->turn;
+  float lanes = (float)nwaves * HVM_WAVE;
+  float itrs = (float)T.itrs;
+  float ws = (float)(T.wave_steps ? T.wave_steps : 1);
+  for (int i = 0; i < HVM_FEAT; ++i) f[i] = 0.0f;
+  f[0] = log2p((float)c->n_in) / 24.0f;
+  f[1] = log2p((float)n_next) / 24.0f;
+  f[2] = fminf(1.0f, (float)n_next / lanes);
+  f[3] = (float)T.lane_steps / (HVM_WAVE * ws);
+  f[4] = log2p(itrs) / 32.0f;
+  f[5] = log2p(dt_us) / 20.0f;
+  f[6] = itrs > 0 ? (float)T.spills / itrs : 0.0f;
+  f[7] = itrs > 0 ? (float)T.defers / itrs : 0.0f;
+  f[8] = fminf(1.0f, (float)T.probes / ws / 4.0f);
+  for (int k = 0; k < HVM_K_KINDS; ++k) f[9 + k] = itrs > 0 ? (float)T.kind[k] / itrs : 0.0f;
+  f[19 + (s->last_action & 3)] = 1.0f;
+  f[23] = log2p((float)c->turns) / 16.0f;
+  f[24] = fminf(1.0f, ws / ((float)nwaves * (float)(c->max_steps ? c->max_steps : 1)));
+  f[25] = 1.0f;
+}
+
+__device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
+
+__global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u32 clock_khz) {
+  __shared__ float f[HVM_FEAT], h1[HVM_HID], h2[HVM_HID], y[HVM_ACTIONS];
+  __shared__ u32 s_go, s_n;
   HvmCtl* c = A.ctl;
-  if (threadIdx.x != 0 || c->done) return;
-  u32 n = c->n_out;
-  c->turns += 1;
-  if (c->err || n > A.bag_cap) {
-    if (n > A.bag_cap) c->err |= HVM_ERR_BAG_OVERFLOW;
-    c->done = 1;
-    return;
+  HvmSched* s = A.sched;
+  const u32 t = threadIdx.x;
+  if (t == 0) {
+    s_go = 0;
+    if (!c->done) {
+      u32 n = c->n_out;
+      c->turns += 1;
+      if (c->err || n > A.bag_cap) {
+        if (n > A.bag_cap) c->err |= HVM_ERR_BAG_OVERFLOW;
+        c->done = 1;
+      } else {
+        // reward of the previous decision: log2 of this turn's interactions per µs
+        unsigned long long now = wall_clock64();
+        float dt_us = s->last_clock ? (float)(now - s->last_clock) * 1000.0f / (float)clock_khz : 0.0f;
+        if (s->trace && s->last_clock && s->trace_len > 0 && s->trace_len <= s->trace_cap) {
+          float* rec = s->trace + (size_t)(s->trace_len - 1) * HVM_TRACE_REC;
+          rec[HVM_FEAT + 1] = __log2f((1.0f + (float)s->turn.itrs) / fmaxf(dt_us, 1.0f));
+          rec[HVM_FEAT + 2] = (float)s->turn.itrs;
+          rec[HVM_FEAT + 3] = dt_us;
+        }
+        sched_features(c, s, n, A.nwaves, dt_us, f);
+        c->n_in = n;
+        c->n_out = 0;
+        c->in_head = min(n, A.nwaves * (u32)HVM_WAVE);
+        c->parity ^= 1u;
+        s_n = n;
+        if (n == 0) c->done = 1;
+        else s_go = 1;
+        s->last_clock = now;
+      }
+      HvmTurnStats z = {};
+      s->turn = z;
+    }
   }
-  c->n_in = n;
-  c->n_out = 0;
-  c->in_head = min(n, A.nwaves * (u32)HVM_WAVE);
-  c->parity ^= 1u;
-  c->max_steps = c->fixed_steps ? c->fixed_steps : turn_policy(n, A.nwaves);
-  if (n == 0) c->done = 1;
+  __syncthreads();
+  if (!s_go) return;
+  const u32 n = s_n;
+  u32 action = plain_action(n, A.nwaves);
+  if (s->mode == HVM_SCHED_LEARNED && s->params) {
+    // 32 -> 64 -> 64 -> 4 MLP, fp16 weights, fp32 accumulation
+    const unsigned short* P = s->params;
+    const unsigned short* W1 = P;
+    const unsigned short* b1 = W1 + HVM_FEAT * HVM_HID;
+    const unsigned short* W2 = b1 + HVM_HID;
+    const unsigned short* b2 = W2 + HVM_HID * HVM_HID;
+    const unsigned short* W3 = b2 + HVM_HID;
+    const unsigned short* b3 = W3 + HVM_HID * HVM_ACTIONS;
+    float acc = h2f(b1[t]);
+    for (int i = 0; i < HVM_FEAT; ++i) acc += f[i] * h2f(W1[i * HVM_HID + t]);
+    h1[t] = fmaxf(acc, 0.0f);
+    __syncthreads();
+    acc = h2f(b2[t]);
+    for (int i = 0; i < HVM_HID; ++i) acc += h1[i] * h2f(W2[i * HVM_HID + t]);
+    h2[t] = fmaxf(acc, 0.0f);
+    __syncthreads();
+    if (t < HVM_ACTIONS) {
+      acc = h2f(b3[t]);
+      for (int i = 0; i < HVM_HID; ++i) acc += h2[i] * h2f(W3[i * HVM_ACTIONS + t]);
+      y[t] = acc;
+    }
+    __syncthreads();
+    action = 0;
+    for (u32 k = 1; k < HVM_ACTIONS; ++k) if (y[k] > y[action]) action = k;
+  } else if (s->mode == HVM_SCHED_EXPLORE) {
+    u32 h = (s->seed * 0x9E3779B9u) ^ (c->turns * 0x85EBCA6Bu);
+    h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12;
+    if (h % 100u < s->explore_pct) action = (h >> 8) & 3u;
+  }
+  if (t == 0) {
+    c->max_steps = c->fixed_steps ? c->fixed_steps : kTurnLen[action];
+    s->last_action = action;
+    if (s->trace && s->trace_len < s->trace_cap) {
+      float* rec = s->trace + (size_t)s->trace_len * HVM_TRACE_REC;
+      for (int i = 0; i < HVM_FEAT; ++i) rec[i] = f[i];
+      rec[HVM_FEAT] = (float)action;
+      rec[HVM_FEAT + 1] = 0.0f;
+      rec[HVM_FEAT + 2] = 0.0f;
+      rec[HVM_FEAT + 3] = 0.0f;
+      s->trace_len += 1;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -625,17 +745,19 @@ extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s) {
   return hipGetLastError();
 }
 
-extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, hipStream_t s) {
-  hipLaunchKernelGGL(hvm_between_kernel, dim3(1), dim3(HVM_WAVE), 0, s, *a);
+extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, uint32_t clock_khz, hipStream_t s) {
+  hipLaunchKernelGGL(hvm_between_kernel, dim3(1), dim3(HVM_WAVE), 0, s, *a, clock_khz);
   return hipGetLastError();
 }
 
+// Turn length of the first turn (the host boots it; later turns are chosen
+// on the device).
 extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves) {
-  uint32_t lanes = nwaves * HVM_WAVE;
-  if (n < nwaves) return 24;
-  if (n < lanes / 4) return 96;
-  return 4096;
+  static const uint32_t len[HVM_ACTIONS] = {24, 96, 512, 4096};
+  return len[plain_action(n, nwaves)];
 }
+
+extern "C" uint32_t hvm_plain_action_host(uint32_t n, uint32_t nwaves) { return plain_action(n, nwaves); }
 
 extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s) {
   hipLaunchKernelGGL(hvm_gather_kernel, dim3(1), dim3(256), 0, s, *g);

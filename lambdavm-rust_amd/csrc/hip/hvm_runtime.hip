@@ -17,7 +17,8 @@
 
 extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s);
 extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s);
-extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, hipStream_t s);
+extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, uint32_t clock_khz, hipStream_t s);
+extern "C" uint32_t hvm_plain_action_host(uint32_t n, uint32_t nwaves);
 extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves);
 extern "C" size_t hvm_turn_lds_bytes(void);
 
@@ -56,6 +57,12 @@ struct hvm_hip_ctx {
   HvmCtl* ctl = nullptr;       // turn control block
   HvmWaveState* wave_state = nullptr;
   HvmDevStats* stats = nullptr;
+  // scheduler
+  HvmSched* sched = nullptr;
+  unsigned short* d_params = nullptr;  // MLP weights (fp16)
+  float* d_trace = nullptr;
+  HvmSched host_sched{};
+  uint32_t clock_khz = 100000;
   // book
   hvm::Book book;
   HvmDevDef* d_defs = nullptr;
@@ -97,8 +104,12 @@ struct hvm_hip_ctx {
       if (per_cu > 16) per_cu = 16;
       nwaves = (u32)(cus * per_cu);
     }
-    node_slice = node_cap / nwaves;
-    vars_slice = vars_cap / nwaves;
+    // node slot 0 and var slots [0, 1 + HVM_HIP_MAX_NETS) (ROOT and the roots
+    // of batched nets) are outside every wave's slice
+    const u32 root_vars = 1 + HVM_HIP_MAX_NETS;
+    if (vars_cap <= root_vars) return fail("vars_cap too small for the root var slots");
+    node_slice = (node_cap - 1) / nwaves;
+    vars_slice = (vars_cap - root_vars) / nwaves;
     if (node_slice < 4 * HVM_WAVE || vars_slice < 4 * HVM_WAVE) return fail("net too small for the wave count");
     if (!ck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
     if (!ck(hipEventCreate(&ev0), "hipEventCreate") || !ck(hipEventCreate(&ev1), "hipEventCreate")) return false;
@@ -109,6 +120,12 @@ struct hvm_hip_ctx {
     if (!ck(hipMalloc(&ctl, sizeof(HvmCtl)), "hipMalloc(ctl)")) return false;
     if (!ck(hipMalloc(&wave_state, nwaves * sizeof(HvmWaveState)), "hipMalloc(wave_state)")) return false;
     if (!ck(hipMalloc(&stats, sizeof(HvmDevStats)), "hipMalloc(stats)")) return false;
+    if (!ck(hipMalloc(&sched, sizeof(HvmSched)), "hipMalloc(sched)")) return false;
+    if (!ck(hipMalloc(&d_params, HVM_MLP_PARAMS * sizeof(unsigned short)), "hipMalloc(params)")) return false;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) == hipSuccess && khz > 0) clock_khz = (uint32_t)khz;
+    memset(&host_sched, 0, sizeof host_sched);
+    host_sched.mode = HVM_SCHED_PLAIN;
     return true;
   }
 
@@ -120,6 +137,9 @@ struct hvm_hip_ctx {
     if (ctl) hipFree(ctl);
     if (wave_state) hipFree(wave_state);
     if (stats) hipFree(stats);
+    if (sched) hipFree(sched);
+    if (d_params) hipFree(d_params);
+    if (d_trace) hipFree(d_trace);
     if (d_defs) hipFree(d_defs);
     if (d_pairs) hipFree(d_pairs);
     if (ev0) hipEventDestroy(ev0);
@@ -206,17 +226,69 @@ struct hvm_hip_ctx {
     return port;
   }
 
-  bool reset_net() {
+  bool reset_heap() {
     if (!ck(hipMemsetAsync(node, 0, (size_t)node_cap * 8, stream), "clear node")) return false;
     if (!ck(hipMemsetAsync(vars, 0, (size_t)vars_cap * 4, stream), "clear vars")) return false;
     if (!ck(hipMemsetAsync(wave_state, 0, nwaves * sizeof(HvmWaveState), stream), "clear cursors")) return false;
-    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return false;
     if (!ck(hipMemsetAsync(ctl, 0, sizeof(HvmCtl), stream), "clear ctl")) return false;
     return ck(hipStreamSynchronize(stream), "reset sync");
   }
 
+  bool reset_net() {
+    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return false;
+    return reset_heap();
+  }
+
+  // Scheduler state for a new normalisation: no previous decision, counters
+  // clear; the trace keeps accumulating until the next eval starts.
+  bool sched_start(u32 n_first, bool new_eval) {
+    u32 len = 0;
+    if (!new_eval && !ck(hipMemcpy(&len, &sched->trace_len, 4, hipMemcpyDeviceToHost), "trace len")) return false;
+    HvmSched s = host_sched;
+    s.trace_len = len;
+    s.last_clock = 0;
+    s.last_action = hvm_plain_action_host(n_first, nwaves);
+    s.params = d_params;
+    s.trace = d_trace;
+    memset(&s.turn, 0, sizeof s.turn);
+    return ck(hipMemcpy(sched, &s, sizeof s, hipMemcpyHostToDevice), "sched upload");
+  }
+
+  bool set_scheduler(u32 mode, u32 seed, u32 explore_pct, const unsigned short* params, u32 trace_cap) {
+    if (mode > HVM_SCHED_EXPLORE) return fail("unknown scheduler mode");
+    if (mode == HVM_SCHED_LEARNED && !params) return fail("learned scheduler needs parameters");
+    if (params && !ck(hipMemcpy(d_params, params, HVM_MLP_PARAMS * sizeof(unsigned short), hipMemcpyHostToDevice),
+                      "params upload")) return false;
+    if (trace_cap != host_sched.trace_cap) {
+      if (d_trace) { hipFree(d_trace); d_trace = nullptr; }
+      if (trace_cap && !ck(hipMalloc(&d_trace, (size_t)trace_cap * HVM_TRACE_REC * sizeof(float)), "hipMalloc(trace)"))
+        return false;
+    }
+    host_sched.mode = mode;
+    host_sched.seed = seed;
+    host_sched.explore_pct = explore_pct;
+    host_sched.trace_cap = trace_cap;
+    return true;
+  }
+
+  int read_trace(float* recs, u32 cap, u32* n) {
+    u32 len = 0;
+    if (!ck(hipMemcpy(&len, &sched->trace_len, 4, hipMemcpyDeviceToHost), "trace len")) return HVM_HIP_E_DEVICE;
+    if (len > host_sched.trace_cap) len = host_sched.trace_cap;
+    u32 k = len < cap ? len : cap;
+    if (k && recs && d_trace &&
+        !ck(hipMemcpy(recs, d_trace, (size_t)k * HVM_TRACE_REC * sizeof(float), hipMemcpyDeviceToHost), "trace copy"))
+      return HVM_HIP_E_DEVICE;
+    if (n) *n = len;
+    return HVM_HIP_OK;
+  }
+
+  bool eval_started = false;  // first boot of an eval resets the trace
+
   // Boots `redex` (src/hvm.c:1204-1207): ROOT := NONE, bag := [redex].
   bool boot(u64 redex) {
+    if (!sched_start(1, !eval_started)) return false;
+    eval_started = true;
     u32 none = HVM_NONE;
     if (!ck(hipMemcpyAsync(vars, &none, 4, hipMemcpyHostToDevice, stream), "boot root")) return false;
     if (!ck(hipMemcpyAsync(bag[0], &redex, 8, hipMemcpyHostToDevice, stream), "boot redex")) return false;
@@ -249,6 +321,8 @@ struct hvm_hip_ctx {
     a.nwaves = nwaves;
     a.wave_state = wave_state;
     a.stats = stats;
+    a.root_vars = 1 + HVM_HIP_MAX_NETS;
+    a.sched = sched;
     return a;
   }
 
@@ -277,7 +351,7 @@ struct hvm_hip_ctx {
         hipEventRecord(tev[2 * i], stream);
         if (!ck(hvm_launch_turn(&a, stream), "launch turn")) return false;
         hipEventRecord(tev[2 * i + 1], stream);
-        if (!ck(hvm_launch_between(&a, stream), "launch between")) return false;
+        if (!ck(hvm_launch_between(&a, clock_khz, stream), "launch between")) return false;
       }
       if (!ck(hipMemcpyAsync(&host_ctl, ctl, sizeof(HvmCtl), hipMemcpyDeviceToHost, stream), "ctl readback")) return false;
       if (!ck(hipStreamSynchronize(stream), "turn sync")) return false;
@@ -444,6 +518,7 @@ struct hvm_hip_ctx {
     kernel_ms = 0;
     turns = 0;
     normalizes = 0;
+    eval_started = false;
     if (!reset_net()) return HVM_HIP_E_DEVICE;
     double t0 = now_s();
     deadline = cfg.time_limit_s ? t0 + cfg.time_limit_s : 0;
@@ -487,6 +562,104 @@ struct hvm_hip_ctx {
     }
     return HVM_HIP_OK;
   }
+
+  // Boots a group of nets: net j's entry def ~ its root var (slot 1 + j).
+  bool boot_batch(const u32* entries, u32 n) {
+    if (!sched_start(n, !eval_started)) return false;
+    eval_started = true;
+    std::vector<u32> roots(n, HVM_NONE);
+    std::vector<u64> rx(n);
+    for (u32 j = 0; j < n; ++j)
+      rx[j] = hvm_new_pair(hvm_new_port(HVM_REF, entries[j]), hvm_new_port(HVM_VAR, 1 + j));
+    if (!ck(hipMemcpyAsync(vars + 1, roots.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream), "boot roots")) return false;
+    if (!ck(hipMemcpyAsync(bag[0], rx.data(), (size_t)n * 8, hipMemcpyHostToDevice, stream), "boot redexes")) return false;
+    HvmCtl c;
+    memset(&c, 0, sizeof c);
+    c.n_in = n;
+    c.in_head = std::min<u32>(n, nwaves * HVM_WAVE);
+    c.fixed_steps = cfg.turn_steps;
+    c.max_steps = cfg.turn_steps ? cfg.turn_steps : hvm_turn_policy_host(n, nwaves);
+    host_ctl = c;
+    if (!ck(hipMemcpyAsync(ctl, &host_ctl, sizeof(HvmCtl), hipMemcpyHostToDevice, stream), "boot ctl")) return false;
+    return ck(hipStreamSynchronize(stream), "boot sync");
+  }
+
+  void fill_out(hvm_hip_out* out, double t0, double t_eval) {
+    HvmDevStats s;
+    memset(&s, 0, sizeof s);
+    hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost);
+    memset(out, 0, sizeof *out);
+    out->itrs = s.itrs;
+    for (int k = 0; k < HVM_HIP_KINDS; ++k) out->kind[k] = s.kind[k];
+    out->bytes = s.bytes;
+    out->steps = s.steps; out->spills = s.spills; out->defers = s.defers;
+    out->retries = s.retries; out->probes = s.probes;
+    out->kernel_ms = kernel_ms;
+    out->eval_ms = (t_eval - t0) * 1e3;
+    out->turns = turns;
+    out->normalizes = normalizes;
+  }
+
+  int eval_batch(const u32* entries, u32 n, const uint64_t* est, hvm_hip_out* outs, hvm_hip_out* total) {
+    if (!loaded) { fail("no book loaded"); return HVM_HIP_E_ARG; }
+    if (!entries || !outs || n == 0) { fail("empty batch"); return HVM_HIP_E_ARG; }
+    for (u32 j = 0; j < n; ++j) {
+      if (entries[j] >= ndefs) { fail("batch entry is not a def"); return HVM_HIP_E_ARG; }
+      memset(&outs[j], 0, sizeof outs[j]);
+    }
+    kernel_ms = 0;
+    turns = 0;
+    normalizes = 0;
+    eval_started = false;
+    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return HVM_HIP_E_DEVICE;
+    double t0 = now_s();
+    deadline = cfg.time_limit_s ? t0 + cfg.time_limit_s : 0;
+    const double budget = 0.5 * (double)node_cap;
+    double t_norm = 0;
+    u32 start = 0;
+    while (start < n) {
+      u32 end = start;
+      double sum = 0;
+      while (end < n && end - start < HVM_HIP_MAX_NETS &&
+             (end == start || !est || sum + (double)est[end] <= budget)) {
+        sum += est ? (double)est[end] : 0.0;
+        ++end;
+      }
+      if (!reset_heap()) return HVM_HIP_E_DEVICE;
+      if (!boot_batch(entries + start, end - start)) return HVM_HIP_E_DEVICE;
+      double ta = now_s();
+      if (!normalize()) return HVM_HIP_E_EVAL;
+      // Per-net readback, as the IO driver does for its one root (src/run.cu:857-925).
+      for (u32 j = start; j < end; ++j) {
+        u32 root = hvm_new_port(HVM_VAR, 1 + (j - start));
+        Ctr c;
+        if (!readback_ctr(peek(root), c)) return HVM_HIP_E_EVAL;
+        if (c.args_len >= 1 && hvm_get_tag(c.args[0]) == HVM_CON && c.tag == 1 && c.args_len == 4) {
+          u64 magic = node_load(hvm_get_val(c.args[0]));
+          if (hvm_get_val(hvm_get_fst(magic)) == hvm_new_u24(0xD0CA11) &&
+              hvm_get_val(hvm_get_snd(magic)) == hvm_new_u24(0xFF1FF1)) {
+            outs[j].err = HVM_HIP_E_IO;
+            continue;
+          }
+        }
+        t_norm += 0;
+        if (cfg.flags & HVM_HIP_NO_READBACK) continue;
+        std::string text;
+        if (!print_result(peek(root), text)) return HVM_HIP_E_EVAL;
+        outs[j].result = dup_cstr(text);
+        std::string canon;
+        try { canon = hvm::canonical_tree_text(text); } catch (std::exception& ex) { canon = "<unparsable>"; }
+        outs[j].canonical = dup_cstr(canon);
+      }
+      t_norm += now_s() - ta;
+      start = end;
+    }
+    if (total) {
+      fill_out(total, t0, now_s());
+      total->wall_s = now_s() - t0;
+    }
+    return HVM_HIP_OK;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -507,6 +680,23 @@ extern "C" hvm_hip_ctx* hvm_hip_create(const hvm_hip_cfg* cfg) {
 extern "C" int hvm_hip_load(hvm_hip_ctx* ctx, const uint32_t* book_buf, size_t n_words) {
   if (!ctx) return HVM_HIP_E_ARG;
   return ctx->load(book_buf, n_words) ? HVM_HIP_OK : HVM_HIP_E_BOOK;
+}
+
+extern "C" int hvm_hip_set_scheduler(hvm_hip_ctx* ctx, uint32_t mode, uint32_t seed, uint32_t explore_pct,
+                                     const uint16_t* params, uint32_t trace_cap) {
+  if (!ctx) return HVM_HIP_E_ARG;
+  return ctx->set_scheduler(mode, seed, explore_pct, params, trace_cap) ? HVM_HIP_OK : HVM_HIP_E_ARG;
+}
+
+extern "C" int hvm_hip_read_trace(hvm_hip_ctx* ctx, float* recs, uint32_t cap, uint32_t* n_recs) {
+  if (!ctx) return HVM_HIP_E_ARG;
+  return ctx->read_trace(recs, cap, n_recs);
+}
+
+extern "C" int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_nets,
+                                  const uint64_t* est_nodes, hvm_hip_out* outs, hvm_hip_out* total) {
+  if (!ctx) return HVM_HIP_E_ARG;
+  return ctx->eval_batch(entries, n_nets, est_nodes, outs, total);
 }
 
 extern "C" int hvm_hip_eval(hvm_hip_ctx* ctx, hvm_hip_out* out) {

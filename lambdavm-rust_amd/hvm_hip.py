@@ -79,6 +79,9 @@ def lib() -> C.CDLL:
     L.hvm_hip_last_error.restype = C.c_char_p
     L.hvm_hip_run.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(_Cfg), C.POINTER(_Out)]
     L.hvm_hip_run.restype = C.c_int
+    L.hvm_hip_eval_batch.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32,
+                                     C.POINTER(C.c_uint64), C.POINTER(_Out), C.POINTER(_Out)]
+    L.hvm_hip_eval_batch.restype = C.c_int
     _lib = L
     return L
 
@@ -176,6 +179,22 @@ class Context:
         if rc != 0:
             raise HvmError(f"eval failed ({STATUS.get(rc, rc)}, err=0x{r.err:x}): " + self.last_error())
         return r
+
+    def eval_batch(self, entries, est_nodes=None):
+        """Evaluates independent nets of the loaded book (hvm_hip_eval_batch).
+        Returns (per-net results, total)."""
+        L = lib()
+        n = len(entries)
+        ents = (C.c_uint32 * n)(*entries)
+        est = (C.c_uint64 * n)(*est_nodes) if est_nodes is not None else None
+        outs = (_Out * n)()
+        total = _Out()
+        rc = L.hvm_hip_eval_batch(self._h, ents, n, est, outs, C.byref(total))
+        per = [_result(outs[i].err, outs[i]) for i in range(n)]
+        tot = _result(rc, total)
+        if rc != 0:
+            raise HvmError(f"eval_batch failed ({STATUS.get(rc, rc)}, err=0x{tot.err:x}): " + self.last_error())
+        return per, tot
 
     def last_error(self) -> str:
         return lib().hvm_hip_last_error(self._h).decode()

@@ -145,9 +145,9 @@ BITONIC_DEFS = """
 """
 
 
-def bitonic(k: int = 16, seed: int = 0) -> Tuple[str, List[int]]:
+def bitonic(k: int = 16, seed: int = 0, chunk: int = 5) -> Tuple[str, List[int]]:
     keys = keys_uniform(1 << k, seed)
-    top, lines = key_tree_defs(keys, "keys")
+    top, lines = key_tree_defs(keys, "keys", chunk)
     code = f"@main = r & @sort ~ ({k} (0 (@{top} r)))\n" + BITONIC_DEFS + "\n".join(lines) + "\n"
     return code, sorted(keys)
 
@@ -264,6 +264,59 @@ def parse_tree_text(s: str):
         return v
 
     return walk()
+
+
+SUM_DEFS = """
+@sum = (?((1 @sum_s) r) r)
+@sum_s = ({a b} r)
+  &! @sum ~ (a $([+] $(c r)))
+  &! @sum ~ (b c)
+"""
+
+
+def batch(seed: int = 0, n: int = 256, chunk: int = 8):
+    """Batched workload: n independent nets drawn from `seed`; the first half
+    are tree sums of depth U[10, 22], the second half bitonic sorts of 2^k keys
+    with k = U[8, 14].  Returns (code, nets) where code is one book holding
+    every net (entry def `@net<i>`) and nets[i] = (generator, params, expected,
+    standalone_code): the same net as a program of its own, whose run-c result
+    is the per-net parity target."""
+    g = splitmix64(seed ^ 0x5EED_BA7C)
+    nets = []
+    body = ["@main = *", SUM_DEFS.strip(), BITONIC_DEFS.strip()]
+    for i in range(n):
+        if i < n // 2:
+            d = 10 + next(g) % 13
+            code_i, expect = sum_tree(d)
+            nets.append(("sum_tree", {"depth": d}, expect, code_i))
+            body.append(f"@net{i} = r & @sum ~ ({d} r)")
+        else:
+            k = 8 + next(g) % 7
+            ks = next(g) & 0xFFFFFFFF
+            code_i, expect = bitonic(k, ks, chunk)
+            nets.append(("bitonic", {"k": k, "seed": ks, "chunk": chunk}, expect, code_i))
+            top, lines = key_tree_defs(keys_uniform(1 << k, ks), f"k{i}", chunk)
+            body.append(f"@net{i} = r & @sort ~ ({k} (0 (@{top} r)))")
+            body.extend(lines)
+    return "\n".join(body) + "\n", nets
+
+
+def def_fids(code: str) -> dict:
+    """fid of every def, as the builder assigns them (main first, then byte order)."""
+    names = sorted({line.split("=", 1)[0].strip()[1:] for line in code.splitlines()
+                    if line.startswith("@") and "=" in line})
+    order = ["main"] + [x for x in names if x != "main"]
+    return {name: i for i, name in enumerate(order)}
+
+
+def batch_estimate(gen: str, params: dict) -> int:
+    """Estimated peak live nodes of one net of the batched workload, used for
+    admission: a tree sum keeps O(2^d) partial sums alive when evaluated
+    breadth-first; a bitonic sort holds its key tree plus one merge level."""
+    if gen == "sum_tree":
+        return 8 << params["depth"]
+    k = params["k"]
+    return 16 * k * (1 << k)
 
 
 CONFIGS = {
