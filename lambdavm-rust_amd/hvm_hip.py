@@ -26,6 +26,13 @@ KIND_BYTES = {"CALL_COPY": 32, "VOID": 0, "ERAS": 32, "ANNI": 48, "COMM": 112,
 
 NO_READBACK = 1
 
+# Scheduler (include/hvm_hip.h)
+SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
+TURN_LENGTHS = (24, 96, 512, 4096)
+MLP_FEAT, MLP_HID, MLP_ACT = 32, 64, 4
+MLP_PARAMS = MLP_FEAT * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_ACT + MLP_ACT
+TRACE_REC = MLP_FEAT + 4
+
 STATUS = {0: "ok", 1: "book failed to load", 2: "IO requested (unsupported)",
           3: "device error", 4: "evaluator error", 5: "bad argument"}
 
@@ -82,6 +89,20 @@ def lib() -> C.CDLL:
     L.hvm_hip_eval_batch.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32,
                                      C.POINTER(C.c_uint64), C.POINTER(_Out), C.POINTER(_Out)]
     L.hvm_hip_eval_batch.restype = C.c_int
+    L.hvm_hip_set_scheduler.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.POINTER(C.c_uint16), C.c_uint32]
+    L.hvm_hip_set_scheduler.restype = C.c_int
+    L.hvm_hip_read_trace.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_uint32)]
+    L.hvm_hip_read_trace.restype = C.c_int
+    F = C.POINTER(C.c_float)
+    U = C.POINTER(C.c_uint32)
+    H = C.POINTER(C.c_uint16)
+    L.hvm_mlp_forward.argtypes = [H, F, C.c_uint32, F]
+    L.hvm_mlp_forward.restype = C.c_int
+    L.hvm_mlp_grad.argtypes = [H, F, U, F, C.c_uint32, F, F]
+    L.hvm_mlp_grad.restype = C.c_int
+    L.hvm_mlp_train.argtypes = [F, F, U, F, C.c_uint32, C.c_uint32, C.c_float, H, F, F]
+    L.hvm_mlp_train.restype = C.c_int
     _lib = L
     return L
 
@@ -196,6 +217,33 @@ class Context:
             raise HvmError(f"eval_batch failed ({STATUS.get(rc, rc)}, err=0x{tot.err:x}): " + self.last_error())
         return per, tot
 
+    def set_scheduler(self, mode: int = SCHED_PLAIN, seed: int = 0, explore_pct: int = 0,
+                      params_fp16=None, trace_cap: int = 0) -> None:
+        """Selects the turn-length scheduler (see include/hvm_hip.h)."""
+        p = None
+        if params_fp16 is not None:
+            p = (C.c_uint16 * MLP_PARAMS)(*[int(v) for v in params_fp16])
+        rc = lib().hvm_hip_set_scheduler(self._h, mode, seed, explore_pct, p, trace_cap)
+        if rc != 0:
+            raise HvmError("set_scheduler failed: " + self.last_error())
+
+    def read_trace(self):
+        """Per-decision records: list of (features[32], action, reward, itrs, us)."""
+        L = lib()
+        n = C.c_uint32()
+        L.hvm_hip_read_trace(self._h, None, 0, C.byref(n))
+        if n.value == 0:
+            return []
+        buf = (C.c_float * (n.value * TRACE_REC))()
+        rc = L.hvm_hip_read_trace(self._h, buf, n.value, C.byref(n))
+        if rc != 0:
+            raise HvmError("read_trace failed: " + self.last_error())
+        out = []
+        for i in range(n.value):
+            r = buf[i * TRACE_REC:(i + 1) * TRACE_REC]
+            out.append((list(r[:MLP_FEAT]), int(r[MLP_FEAT]), r[MLP_FEAT + 1], r[MLP_FEAT + 2], r[MLP_FEAT + 3]))
+        return out
+
     def last_error(self) -> str:
         return lib().hvm_hip_last_error(self._h).decode()
 
@@ -232,3 +280,54 @@ def run_hip(words, **cfg) -> RunResult:
 
 def run_code(code: str, **cfg) -> RunResult:
     return run_hip(compile_book(code), **cfg)
+
+
+# ---------------------------------------------------------------------------
+# Scheduler MLP (training runs on the GPU through the library's HIP kernels)
+# ---------------------------------------------------------------------------
+
+def _farr(xs):
+    return (C.c_float * len(xs))(*xs)
+
+
+def mlp_forward(params_fp16, x_rows):
+    """x_rows: list of 32-float rows → list of 4-float predictions."""
+    n = len(x_rows)
+    p = (C.c_uint16 * MLP_PARAMS)(*params_fp16)
+    x = _farr([v for row in x_rows for v in row])
+    y = (C.c_float * (n * MLP_ACT))()
+    rc = lib().hvm_mlp_forward(p, x, n, y)
+    if rc != 0:
+        raise HvmError(f"hvm_mlp_forward failed: {STATUS.get(rc, rc)}")
+    return [list(y[i * MLP_ACT:(i + 1) * MLP_ACT]) for i in range(n)]
+
+
+def mlp_grad(params_fp16, x_rows, actions, rewards):
+    """Mean gradient (packed like the params) and mean loss, on the GPU."""
+    n = len(x_rows)
+    p = (C.c_uint16 * MLP_PARAMS)(*params_fp16)
+    x = _farr([v for row in x_rows for v in row])
+    a = (C.c_uint32 * n)(*actions)
+    r = _farr(rewards)
+    g = (C.c_float * MLP_PARAMS)()
+    loss = C.c_float()
+    rc = lib().hvm_mlp_grad(p, x, a, r, n, g, C.byref(loss))
+    if rc != 0:
+        raise HvmError(f"hvm_mlp_grad failed: {STATUS.get(rc, rc)}")
+    return list(g), loss.value
+
+
+def mlp_train(params_fp32, x_rows, actions, rewards, epochs=300, lr=3e-3):
+    """Full-batch Adam on the GPU.  Returns (fp16 params, fp32 params, loss)."""
+    n = len(x_rows)
+    pin = _farr(params_fp32)
+    x = _farr([v for row in x_rows for v in row])
+    a = (C.c_uint32 * n)(*actions)
+    r = _farr(rewards)
+    out16 = (C.c_uint16 * MLP_PARAMS)()
+    out32 = (C.c_float * MLP_PARAMS)()
+    loss = C.c_float()
+    rc = lib().hvm_mlp_train(pin, x, a, r, n, epochs, lr, out16, out32, C.byref(loss))
+    if rc != 0:
+        raise HvmError(f"hvm_mlp_train failed: {STATUS.get(rc, rc)}")
+    return list(out16), list(out32), loss.value

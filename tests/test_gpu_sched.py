@@ -1,0 +1,98 @@
+"""GPU tests of the scheduler MLP kernels, the scheduler modes and batching.
+
+* MLP forward and gradient (HIP, fp16 weights, fp32 math) against the float32
+  numpy reference in tests/mlp_ref.py (itself checked against finite
+  differences in test_sched_cpu.py); tolerance rtol 2e-3 / atol 2e-5;
+* every scheduler mode (plain, explore with several seeds, learned with
+  random and with trained weights) gives the same normal form and ITRS;
+* the batched workload: every net's normal form equals the reference run-c
+  result of that net alone (tests/golden/batch.json), and the total ITRS is
+  their sum.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import mlp_ref
+import programs as P
+import sched as S
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 2e-3, 2e-5
+
+
+def rand_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.25, sum(mlp_ref.SIZES)).astype(np.float32)
+    bits = mlp_ref.to_half_bits(p)
+    X = rng.random((n, mlp_ref.F)).astype(np.float32)
+    A = rng.integers(0, mlp_ref.K, n).astype(np.uint32)
+    R = rng.normal(0, 1, n).astype(np.float32)
+    return bits, X, A, R
+
+
+def test_mlp_forward_matches_reference(hip):
+    bits, X, A, R = rand_problem(300, 1)
+    y = np.array(hip.mlp_forward(bits.tolist(), X.tolist()), dtype=np.float32)
+    ref, _ = mlp_ref.forward(mlp_ref.from_half_bits(bits), X)
+    assert np.allclose(y, ref, rtol=RTOL, atol=ATOL)
+
+
+@pytest.mark.parametrize("n,seed", [(1, 2), (7, 3), (513, 4)])
+def test_mlp_gradient_matches_reference(hip, n, seed):
+    bits, X, A, R = rand_problem(n, seed)
+    g, loss = hip.mlp_grad(bits.tolist(), X.tolist(), A.tolist(), R.tolist())
+    ref_loss, ref_g = mlp_ref.loss_and_grad(mlp_ref.from_half_bits(bits), X, A, R)
+    assert np.allclose(np.array(g, dtype=np.float32), ref_g, rtol=RTOL, atol=ATOL)
+    assert abs(loss - ref_loss) <= RTOL * max(1.0, abs(ref_loss))
+
+
+def test_mlp_training_lowers_loss(hip):
+    rng = np.random.default_rng(5)
+    X = rng.random((256, mlp_ref.F)).astype(np.float32)
+    A = rng.integers(0, mlp_ref.K, 256)
+    R = (X[:, 0] * 2 - X[:, 1] + 0.5 * A).astype(np.float32)   # learnable target
+    p0 = S.init_params(0)
+    _, l0 = hip.mlp_grad(mlp_ref.to_half_bits(p0).tolist(), X.tolist(), A.tolist(), R.tolist())
+    p16, p32, l1 = hip.mlp_train(p0, X.tolist(), A.tolist(), R.tolist(), epochs=300, lr=3e-3)
+    assert l1 < 0.25 * l0
+
+
+def test_scheduler_never_changes_results(hip):
+    code, expect = P.bitonic(9, 4)
+    words = hip.compile_book(code)
+    seen = set()
+    with hip.Context(time_limit_s=120) as ctx:
+        ctx.load(words)
+        runs = [(hip.SCHED_PLAIN, 0, None)] + [(hip.SCHED_EXPLORE, s, None) for s in (1, 2, 3)]
+        rnd = mlp_ref.to_half_bits(np.random.default_rng(9).normal(0, 0.3, sum(mlp_ref.SIZES))).tolist()
+        runs.append((hip.SCHED_LEARNED, 0, rnd))
+        for mode, seed, params in runs:
+            ctx.set_scheduler(mode, seed=seed, explore_pct=50, params_fp16=params, trace_cap=4096)
+            r = ctx.eval()
+            assert P.flatten_tree(P.parse_tree_text(r.result)) == expect
+            seen.add((r.itrs, r.canonical))
+            tr = ctx.read_trace()
+            assert len(tr) >= 1
+            assert all(0 <= rec[1] < 4 for rec in tr)
+    assert len(seen) == 1
+
+
+def test_batch_matches_per_net_reference(hip):
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "batch.json")))
+    code, nets = P.batch(g["seed"], g["n"])
+    fids = P.def_fids(code)
+    entries = [fids[f"net{i}"] for i in range(len(nets))]
+    est = [P.batch_estimate(gen, params) for gen, params, _, _ in nets]
+    with hip.Context(time_limit_s=300) as ctx:
+        ctx.load(hip.compile_book(code))
+        per, total = ctx.eval_batch(entries, est)
+    assert total.itrs == g["total_itrs"]
+    for i, (row, r) in enumerate(zip(g["nets"], per)):
+        assert r.err == 0, i
+        assert hashlib.sha256(r.canonical.encode()).hexdigest() == row["nf_sha256"], i
