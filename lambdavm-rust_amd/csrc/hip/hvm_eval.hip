@@ -106,7 +106,7 @@ struct WaveLds {
   u32 tbl_v[HVM_TBL_VARS];
   u64 inbox[HVM_WAVE];
   u32 target[HVM_WAVE];
-  u32 cursor;                   // allocator scratch
+  u32 cursor_n, cursor_v;       // allocator scratch
 };
 
 struct Lane {
@@ -174,37 +174,60 @@ __device__ __forceinline__ void link(WaveLds& L, Lane& ln, const HvmTurnArgs& A,
 // `need` and `cur` are wave-uniform.
 // ---------------------------------------------------------------------------
 
+// One 64-slot window of one buffer: records free slots into tbl[got...],
+// advances the cursor (to just after the last slot taken when the window
+// satisfies the request).  Returns the number of slots still wanted.
 template <typename T>
-__device__ __forceinline__ bool wave_alloc(WaveLds& L, T* buf, u32 lo, u32 len, u32& cur, u32 need,
-                                           u32* tbl, unsigned long long& probes, u32 reserved) {
-  u32 lane = lane_id();
-  u32 got = 0;
-  u32 scanned = 0;
-  while (got < need) {
-    u32 rel = (cur - lo + lane) % len;
-    u32 idx = lo + rel;
-    T v = __hip_atomic_load(&buf[idx], __ATOMIC_RELAXED, AGENT);
-    bool free_ = (v == 0) && (idx >= reserved);
-    u64 m = __ballot(free_);
-    u32 r = rank_in(m);
-    u32 nfree = __popcll(m);
-    u32 want = need - got;
-    if (free_ && r < want) tbl[got + r] = idx;
-    if (nfree >= want) {
-      // next scan starts just after the last slot taken
-      if (free_ && r == want - 1) L.cursor = lo + ((rel + 1) % len);
-      __syncthreads();
-      cur = L.cursor;
-      got = need;
-    } else {
-      got += nfree;
-      cur = lo + ((cur - lo + HVM_WAVE) % len);
+__device__ __forceinline__ u32 alloc_window(T v, u32 idx, u32 rel, u32 lo, u32 len, u32 reserved, u32& cur,
+                                            u32& got, u32 need, u32* tbl, u32* cursor_slot) {
+  bool free_ = (v == 0) && (idx >= reserved);
+  u64 m = __ballot(free_);
+  u32 r = rank_in(m);
+  u32 nfree = __popcll(m);
+  u32 want = need - got;
+  if (free_ && r < want) tbl[got + r] = idx;
+  if (nfree >= want) {
+    if (free_ && r == want - 1) *cursor_slot = lo + ((rel + 1) % len);
+    got = need;
+    return 0;
+  }
+  got += nfree;
+  cur = lo + ((cur - lo + HVM_WAVE) % len);
+  return need - got;
+}
+
+// Allocates need_n node slots and need_v var slots for the whole wave.  Each
+// round issues one coalesced probe load per buffer that still needs slots, so
+// both buffers are scanned concurrently.  need_* and cursors are uniform.
+__device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u32 n_lo, u32 v_lo, u32& n_cur,
+                                            u32& v_cur, u32 need_n, u32 need_v, u64& probes, u32& oom) {
+  const u32 lane = lane_id();
+  u32 got_n = 0, got_v = 0, scan_n = 0, scan_v = 0;
+  L.cursor_n = n_cur;
+  L.cursor_v = v_cur;
+  while (got_n < need_n || got_v < need_v) {
+    const bool pn = got_n < need_n, pv = got_v < need_v;
+    u32 rn = (n_cur - n_lo + lane) % A.node_slice, in_ = n_lo + rn;
+    u32 rv = (v_cur - v_lo + lane) % A.vars_slice, iv = v_lo + rv;
+    u64 vn = pn ? __hip_atomic_load(&A.node[in_], __ATOMIC_RELAXED, AGENT) : 1ull;
+    u32 vv = pv ? __hip_atomic_load(&A.vars[iv], __ATOMIC_RELAXED, AGENT) : 1u;
+    if (pn) {
+      alloc_window(vn, in_, rn, n_lo, A.node_slice, 1u, n_cur, got_n, need_n, L.tbl_n, &L.cursor_n);
+      scan_n += HVM_WAVE;
+      probes++;
+      if (got_n < need_n && scan_n >= A.node_slice + HVM_WAVE) { oom = HVM_ERR_OOM_NODE; return false; }
     }
-    probes++;
-    scanned += HVM_WAVE;
-    if (got < need && scanned >= len + HVM_WAVE) return false;
+    if (pv) {
+      alloc_window(vv, iv, rv, v_lo, A.vars_slice, A.root_vars, v_cur, got_v, need_v, L.tbl_v, &L.cursor_v);
+      scan_v += HVM_WAVE;
+      probes++;
+      if (got_v < need_v && scan_v >= A.vars_slice + HVM_WAVE) { oom = HVM_ERR_OOM_VARS; return false; }
+    }
   }
   __syncthreads();
+  // a satisfied request left its cursor just after its last slot
+  if (need_n) n_cur = L.cursor_n;
+  if (need_v) v_cur = L.cursor_v;
   return true;
 }
 
@@ -328,28 +351,25 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     u32 need_v = wave_max(run ? off_v + nv : 0);
     if (run) ln.top--;
 
-    // ---- P2: allocation ----
-    if (need_n) {
-      if (!wave_alloc(L, A.node, n_lo, A.node_slice, n_cur, need_n, L.tbl_n, probes, 1u)) {
-        atomicOr(&A.ctl->err, (u32)HVM_ERR_OOM_NODE);
-        if (run) ln.top++;
-        break;
-      }
-    }
-    if (need_v) {
-      if (!wave_alloc(L, A.vars, v_lo, A.vars_slice, v_cur, need_v, L.tbl_v, probes, A.root_vars)) {
-        atomicOr(&A.ctl->err, (u32)HVM_ERR_OOM_VARS);
-        if (run) ln.top++;
-        break;
-      }
-    }
-
-    // ---- P3: takes ----
+    // ---- P3: takes, issued before the allocator's probes so that both
+    //      round trips overlap ----
     bool take_a = run && (rule == HVM_ANNI || rule == HVM_COMM);
     bool take_b = run && (rule == HVM_ERAS || rule == HVM_ANNI || rule == HVM_COMM ||
                           rule == HVM_OPER || rule == HVM_SWIT || (rule == HVM_CALL && copy));
     u64 NA = take_a ? node_take(A, hvm_get_val(a)) : 0ull;
     u64 NB = take_b ? node_take(A, hvm_get_val(b)) : 0ull;
+
+    // ---- P2: allocation (wave-cooperative, nodes and vars together) ----
+    if (need_n || need_v) {
+      u32 oom = 0;
+      if (!wave_alloc2(L, A, n_lo, v_lo, n_cur, v_cur, need_n, need_v, probes, oom)) {
+        // out of slots in this wave's slice: the evaluation stops with an error
+        atomicOr(&A.ctl->err, oom);
+        if (run) ln.top++;
+        break;
+      }
+    }
+
     if ((take_a && NA == 0) || (take_b && NB == 0)) {
       // A node this redex points to is not visible yet: put everything back
       // and retry the redex on a later step (src/hvm.c:1112-1114).
