@@ -39,6 +39,22 @@ def test_cli_compile_matches_library(hip, tmp_path):
     assert words == hip.compile_book(src.read_text())
 
 
+def test_gen_hip_emits_a_buildable_program(hip, tmp_path):
+    exe = os.path.join(PKG, "bin", "hvm")
+    if not os.access(exe, os.X_OK):
+        pytest.skip("CLI not built")
+    src = tmp_path / "p.hvm"
+    src.write_text("@main = r & @f ~ (3 r)\n@f = ({a b} c) & a ~ $([+] $(b c))\n")
+    out = subprocess.run([exe, "gen-hip", str(src)], capture_output=True, text=True, check=True).stdout
+    body = out.split("{", 1)[1].split("}", 1)[0]
+    words = [int(x, 16) for x in re.findall(r"0x([0-9a-f]{8})", body)]
+    assert words == hip.compile_book(src.read_text())
+    cpp = tmp_path / "gen.cpp"
+    cpp.write_text(out)
+    subprocess.run(["g++", str(cpp), "-I", os.path.join(ROOT, "include"), "-L", os.path.join(PKG, "lib"),
+                    "-lhvm_hip", "-o", str(tmp_path / "gen")], check=True)
+
+
 def test_cli_reports_missing_main(tmp_path):
     exe = os.path.join(PKG, "bin", "hvm")
     if not os.access(exe, os.X_OK):
