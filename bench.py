@@ -55,7 +55,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--k", type=int, default=16, help="log2 of the key count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (torch), to calibrate "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE against a known byte count")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(this evaluator runs a net on a single device)", file=sys.stderr)
