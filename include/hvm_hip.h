@@ -49,6 +49,9 @@ typedef struct hvm_hip_cfg {
 } hvm_hip_cfg;
 
 #define HVM_HIP_NO_READBACK 1u
+/* Order node/var creations before the links by waiting for the creation
+ * stores to complete instead of an agent-scope release fence. */
+#define HVM_HIP_WAITCNT_FENCE 2u
 
 /* Interaction kinds, in the order of hvm_hip_out.kind[]. */
 enum {

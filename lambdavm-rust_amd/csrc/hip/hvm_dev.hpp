@@ -140,7 +140,12 @@ struct HvmTurnArgs {
   HvmDevStats* stats;
   uint32_t root_vars;              // var slots [0, root_vars) are roots, never allocated
   HvmSched* sched;
+  uint32_t flags;                  // HVM_TURN_* behaviour switches
 };
+
+// Release before the links: the default is the agent-scope release fence;
+// this flag replaces it by a wait for the completion of the creation stores.
+#define HVM_TURN_WAITCNT_FENCE 1u
 
 struct HvmGatherArgs {
   const unsigned long long* node;

@@ -427,7 +427,13 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     }
 
     // ---- P5: publish creations before any link can expose them ----
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (A.flags & HVM_TURN_WAITCNT_FENCE) {
+      // Node/var creations are device-scope (write-through) stores: waiting
+      // for their completion orders them before the link exchanges.
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    }
 
     // ---- P6: links ----
     if (run) {

@@ -323,6 +323,7 @@ struct hvm_hip_ctx {
     a.stats = stats;
     a.root_vars = 1 + HVM_HIP_MAX_NETS;
     a.sched = sched;
+    a.flags = (cfg.flags & HVM_HIP_WAITCNT_FENCE) ? HVM_TURN_WAITCNT_FENCE : 0u;
     return a;
   }
 

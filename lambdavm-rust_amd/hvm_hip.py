@@ -25,6 +25,7 @@ KIND_BYTES = {"CALL_COPY": 32, "VOID": 0, "ERAS": 32, "ANNI": 48, "COMM": 112,
               "OPER1": 32, "OPER2": 40, "SWIT0": 32, "SWITN": 40}
 
 NO_READBACK = 1
+WAITCNT_FENCE = 2
 
 # Scheduler (include/hvm_hip.h)
 SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
