@@ -46,12 +46,16 @@ typedef struct hvm_hip_cfg {
   uint32_t turn_steps;  /* steps per turn (0: adaptive)                       */
   uint32_t flags;       /* HVM_HIP_NO_READBACK: skip the result text          */
   uint32_t time_limit_s;/* abandon evaluation after this many seconds (0: none) */
+  uint32_t idle_pct;    /* end a turn early once this % of waves are idle
+                           (0: 50; above 100: never end early)                */
 } hvm_hip_cfg;
 
 #define HVM_HIP_NO_READBACK 1u
-/* Order node/var creations before the links by waiting for the creation
- * stores to complete instead of an agent-scope release fence. */
-#define HVM_HIP_WAITCNT_FENCE 2u
+/* Node/var creations are ordered before the links that publish them by
+ * waiting for the creation stores (device-scope, write-through) to complete.
+ * This flag uses a full agent-scope release fence (with L2 writeback)
+ * instead. */
+#define HVM_HIP_FENCE_RELEASE 2u
 
 /* Interaction kinds, in the order of hvm_hip_out.kind[]. */
 enum {

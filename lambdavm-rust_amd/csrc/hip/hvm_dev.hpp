@@ -60,8 +60,10 @@ enum {
 
 // Per-wave persistent allocator state (cursor into the wave's slices).
 struct HvmWaveState {
-  uint32_t node_cur;
+  uint32_t node_cur;      // scan cursor in the wave's own slices
   uint32_t vars_cur;
+  uint32_t node_ovf;      // scan cursor in the shared overflow pools
+  uint32_t vars_ovf;
 };
 
 // Turn control block, in device memory.  The turn kernel reads it at start;
@@ -76,7 +78,8 @@ struct HvmCtl {
   uint32_t turns;       // turns completed
   uint32_t done;        // set once a turn leaves no redex (or on error)
   uint32_t fixed_steps; // if nonzero, every turn runs this many steps
-  uint32_t pad[7];
+  uint32_t idle_waves;  // waves of the current turn that ran out of work
+  uint32_t pad[6];
 };
 
 // ---------------------------------------------------------------------------
@@ -126,6 +129,7 @@ struct HvmTurnArgs {
   uint32_t* vars;
   uint32_t node_cap, vars_cap;     // slots (slot 0 never allocated)
   uint32_t node_slice, vars_slice; // slots per wave
+  uint32_t node_pool_lo, vars_pool_lo; // shared overflow pools: [pool_lo, cap)
   // book
   const HvmDevDef* defs;
   const unsigned long long* pairs;
@@ -141,6 +145,7 @@ struct HvmTurnArgs {
   uint32_t root_vars;              // var slots [0, root_vars) are roots, never allocated
   HvmSched* sched;
   uint32_t flags;                  // HVM_TURN_* behaviour switches
+  uint32_t idle_quit;              // a turn ends once this many waves are idle (0: never early)
 };
 
 // Release before the links: the default is the agent-scope release fence;

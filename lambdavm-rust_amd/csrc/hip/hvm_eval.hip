@@ -174,60 +174,93 @@ __device__ __forceinline__ void link(WaveLds& L, Lane& ln, const HvmTurnArgs& A,
 // `need` and `cur` are wave-uniform.
 // ---------------------------------------------------------------------------
 
-// One 64-slot window of one buffer: records free slots into tbl[got...],
-// advances the cursor (to just after the last slot taken when the window
-// satisfies the request).  Returns the number of slots still wanted.
+// Scan state of one buffer.  A wave first scans its own slice, where only it
+// turns zero slots non-zero, so a zero read there is free.  When its slice
+// has no room it scans the shared overflow pool, where every wave allocates:
+// there a slot is claimed by compare-and-swap from zero.
+struct Scan {
+  u32 lo, len;        // range scanned (own slice, then the pool)
+  u32 cur;            // next window start
+  u32 got, need;      // slots found / wanted
+  u32 scanned;        // slots looked at in the current range
+  bool pool;          // scanning the overflow pool
+};
+
+__device__ __forceinline__ u32 scan_idx(const Scan& s, u32 lane, u32& rel) {
+  rel = (s.cur - s.lo + lane) % s.len;
+  return s.lo + rel;
+}
+
+// Processes one 64-slot window (value v at idx for this lane).  Records free
+// slots into tbl[got...]; when the window satisfies the request the lane
+// holding the last slot taken writes the next cursor to *cursor_slot.
 template <typename T>
-__device__ __forceinline__ u32 alloc_window(T v, u32 idx, u32 rel, u32 lo, u32 len, u32 reserved, u32& cur,
-                                            u32& got, u32 need, u32* tbl, u32* cursor_slot) {
-  bool free_ = (v == 0) && (idx >= reserved);
-  u64 m = __ballot(free_);
+__device__ __forceinline__ void scan_window(T* buf, Scan& s, T v, u32 idx, u32 rel, u32 reserved, T claim,
+                                            u32* tbl, u32* cursor_slot) {
+  bool cand = (v == 0) && (idx >= reserved);
+  if (s.pool && cand) cand = atomicCAS(&buf[idx], (T)0, claim) == (T)0;
+  u64 m = __ballot(cand);
   u32 r = rank_in(m);
   u32 nfree = __popcll(m);
-  u32 want = need - got;
-  if (free_ && r < want) tbl[got + r] = idx;
+  u32 want = s.need - s.got;
+  if (cand && r < want) tbl[s.got + r] = idx;
+  // claimed pool slots beyond the request go back to the pool
+  if (s.pool && cand && r >= want) __hip_atomic_store(&buf[idx], (T)0, __ATOMIC_RELAXED, AGENT);
   if (nfree >= want) {
-    if (free_ && r == want - 1) *cursor_slot = lo + ((rel + 1) % len);
-    got = need;
-    return 0;
+    if (cand && r == want - 1) *cursor_slot = s.lo + ((rel + 1) % s.len);
+    s.got = s.need;
+  } else {
+    s.got += nfree;
+    s.cur = s.lo + ((s.cur - s.lo + HVM_WAVE) % s.len);
+    s.scanned += HVM_WAVE;
   }
-  got += nfree;
-  cur = lo + ((cur - lo + HVM_WAVE) % len);
-  return need - got;
+}
+
+// After a window that left the request open: moves from an exhausted slice
+// to the pool; returns false when the pool is exhausted too.
+__device__ __forceinline__ bool scan_advance(Scan& s, u32 pool_lo, u32 pool_len, u32 pool_cur) {
+  if (s.got >= s.need || s.scanned < s.len + HVM_WAVE) return true;
+  if (s.pool || pool_len < HVM_WAVE) return false;
+  s.pool = true;
+  s.lo = pool_lo;
+  s.len = pool_len;
+  s.cur = pool_cur;
+  s.scanned = 0;
+  return true;
 }
 
 // Allocates need_n node slots and need_v var slots for the whole wave.  Each
 // round issues one coalesced probe load per buffer that still needs slots, so
-// both buffers are scanned concurrently.  need_* and cursors are uniform.
-__device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u32 n_lo, u32 v_lo, u32& n_cur,
-                                            u32& v_cur, u32 need_n, u32 need_v, u64& probes, u32& oom) {
+// both buffers are scanned concurrently.  All arguments are wave-uniform.
+__device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u32 n_lo, u32 v_lo,
+                                            HvmWaveState& cur, u32 need_n, u32 need_v, u64& probes, u32& oom) {
   const u32 lane = lane_id();
-  u32 got_n = 0, got_v = 0, scan_n = 0, scan_v = 0;
-  L.cursor_n = n_cur;
-  L.cursor_v = v_cur;
-  while (got_n < need_n || got_v < need_v) {
-    const bool pn = got_n < need_n, pv = got_v < need_v;
-    u32 rn = (n_cur - n_lo + lane) % A.node_slice, in_ = n_lo + rn;
-    u32 rv = (v_cur - v_lo + lane) % A.vars_slice, iv = v_lo + rv;
+  Scan sn = {n_lo, A.node_slice, cur.node_cur, 0, need_n, 0, false};
+  Scan sv = {v_lo, A.vars_slice, cur.vars_cur, 0, need_v, 0, false};
+  L.cursor_n = cur.node_cur;
+  L.cursor_v = cur.vars_cur;
+  while (sn.got < sn.need || sv.got < sv.need) {
+    const bool pn = sn.got < sn.need, pv = sv.got < sv.need;
+    u32 rn, rv;
+    u32 in_ = scan_idx(sn, lane, rn);
+    u32 iv = scan_idx(sv, lane, rv);
     u64 vn = pn ? __hip_atomic_load(&A.node[in_], __ATOMIC_RELAXED, AGENT) : 1ull;
     u32 vv = pv ? __hip_atomic_load(&A.vars[iv], __ATOMIC_RELAXED, AGENT) : 1u;
     if (pn) {
-      alloc_window(vn, in_, rn, n_lo, A.node_slice, 1u, n_cur, got_n, need_n, L.tbl_n, &L.cursor_n);
-      scan_n += HVM_WAVE;
+      scan_window(A.node, sn, vn, in_, rn, 1u, ~0ull, L.tbl_n, &L.cursor_n);
       probes++;
-      if (got_n < need_n && scan_n >= A.node_slice + HVM_WAVE) { oom = HVM_ERR_OOM_NODE; return false; }
+      if (!scan_advance(sn, A.node_pool_lo, A.node_cap - A.node_pool_lo, cur.node_ovf)) { oom = HVM_ERR_OOM_NODE; return false; }
     }
     if (pv) {
-      alloc_window(vv, iv, rv, v_lo, A.vars_slice, A.root_vars, v_cur, got_v, need_v, L.tbl_v, &L.cursor_v);
-      scan_v += HVM_WAVE;
+      scan_window(A.vars, sv, vv, iv, rv, A.root_vars, (u32)HVM_NONE, L.tbl_v, &L.cursor_v);
       probes++;
-      if (got_v < need_v && scan_v >= A.vars_slice + HVM_WAVE) { oom = HVM_ERR_OOM_VARS; return false; }
+      if (!scan_advance(sv, A.vars_pool_lo, A.vars_cap - A.vars_pool_lo, cur.vars_ovf)) { oom = HVM_ERR_OOM_VARS; return false; }
     }
   }
   __syncthreads();
   // a satisfied request left its cursor just after its last slot
-  if (need_n) n_cur = L.cursor_n;
-  if (need_v) v_cur = L.cursor_v;
+  if (need_n) { if (sn.pool) cur.node_ovf = L.cursor_n; else cur.node_cur = L.cursor_n; }
+  if (need_v) { if (sv.pool) cur.vars_ovf = L.cursor_v; else cur.vars_cur = L.cursor_v; }
   return true;
 }
 
@@ -259,10 +292,14 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
 
   // Slices start after the reserved slots: node slot 0, var slots [0, root_vars).
   const u32 n_lo = 1 + w * A.node_slice, v_lo = A.root_vars + w * A.vars_slice;
-  u32 n_cur = A.wave_state[w].node_cur;
-  u32 v_cur = A.wave_state[w].vars_cur;
-  if (n_cur < n_lo || n_cur >= n_lo + A.node_slice) n_cur = n_lo;
-  if (v_cur < v_lo || v_cur >= v_lo + A.vars_slice) v_cur = v_lo;
+  HvmWaveState ws = A.wave_state[w];
+  if (ws.node_cur < n_lo || ws.node_cur >= n_lo + A.node_slice) ws.node_cur = n_lo;
+  if (ws.vars_cur < v_lo || ws.vars_cur >= v_lo + A.vars_slice) ws.vars_cur = v_lo;
+  // overflow-pool cursors start spread out, one region per wave
+  if (ws.node_ovf < A.node_pool_lo || ws.node_ovf >= A.node_cap)
+    ws.node_ovf = A.node_pool_lo + (u32)(((u64)(A.node_cap - A.node_pool_lo) * w) / A.nwaves);
+  if (ws.vars_ovf < A.vars_pool_lo || ws.vars_ovf >= A.vars_cap)
+    ws.vars_ovf = A.vars_pool_lo + (u32)(((u64)(A.vars_cap - A.vars_pool_lo) * w) / A.nwaves);
 
   // Initial spread: lane l of wave w takes item w + l*W, so a small bag is
   // spread one redex per wave before any wave takes a second one.
@@ -310,10 +347,19 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
 
     bool has = ln.top != ln.bot;
     if (!__ballot(has)) {
-      if (!more) break;
+      if (!more) {
+        // out of work: count this wave idle, so that busy waves end the turn
+        // early and their redexes get redistributed
+        if (lane == 0) atomicAdd(&A.ctl->idle_waves, 1u);
+        break;
+      }
       continue;
     }
-    if ((step & 63) == 0 && __hip_atomic_load(&A.ctl->err, __ATOMIC_RELAXED, AGENT) != 0) break;
+    if ((step & 15) == 15) {
+      u32 idle = __hip_atomic_load(&A.ctl->idle_waves, __ATOMIC_RELAXED, AGENT);
+      if (A.idle_quit && idle >= A.idle_quit) break;
+      if ((step & 63) == 63 && __hip_atomic_load(&A.ctl->err, __ATOMIC_RELAXED, AGENT) != 0) break;
+    }
     steps++;
     lane_steps += __popcll(__ballot(has));
 
@@ -362,8 +408,8 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     // ---- P2: allocation (wave-cooperative, nodes and vars together) ----
     if (need_n || need_v) {
       u32 oom = 0;
-      if (!wave_alloc2(L, A, n_lo, v_lo, n_cur, v_cur, need_n, need_v, probes, oom)) {
-        // out of slots in this wave's slice: the evaluation stops with an error
+      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom)) {
+        // out of slots in the slice and the pool: the evaluation stops with an error
         atomicOr(&A.ctl->err, oom);
         if (run) ln.top++;
         break;
@@ -375,6 +421,9 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       // and retry the redex on a later step (src/hvm.c:1112-1114).
       if (take_a && NA) node_put(A, hvm_get_val(a), NA);
       if (take_b && NB) node_put(A, hvm_get_val(b), NB);
+      // slots allocated for this interaction go back (they may be pool claims)
+      for (u32 i = 0; i < nn; ++i) node_put(A, L.tbl_n[off_n + i], 0ull);
+      for (u32 i = 0; i < nv; ++i) vars_put(A, L.tbl_v[off_v + i], 0u);
       ln.top++;
       run = false;
       retries++;
@@ -386,7 +435,10 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     if (run && rule == HVM_OPER) {
       B2 = enter(A, B2);
       oper1 = hvm_get_tag(B1) == HVM_NUM;
-      if (oper1) cv = hvm_operate(hvm_get_val(a), hvm_get_val(B1));
+      if (oper1) {
+        cv = hvm_operate(hvm_get_val(a), hvm_get_val(B1));
+        node_put(A, L.tbl_n[off_n], 0ull);  // the node reserved for OPER-2 is not needed
+      }
     }
 
     // ---- P4: creation ----
@@ -564,8 +616,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       if (defers) atomicAdd(&A.stats->defers, defers);
       if (retries) atomicAdd(&A.stats->retries, retries);
       if (probes) atomicAdd(&A.stats->probes, probes);
-      A.wave_state[w].node_cur = n_cur;
-      A.wave_state[w].vars_cur = v_cur;
+      A.wave_state[w] = ws;
       // per-turn counters for the scheduler
       HvmTurnStats* T = &A.sched->turn;
       if (s_itrs) atomicAdd(&T->itrs, s_itrs);
@@ -702,6 +753,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
         sched_features(c, s, n, A.nwaves, dt_us, f);
         c->n_in = n;
         c->n_out = 0;
+        c->idle_waves = 0;
         c->in_head = min(n, A.nwaves * (u32)HVM_WAVE);
         c->parity ^= 1u;
         s_n = n;

@@ -51,6 +51,7 @@ struct hvm_hip_ctx {
   u32* vars = nullptr;
   u32 node_cap = 0, vars_cap = 0;
   u32 nwaves = 0, node_slice = 0, vars_slice = 0;
+  u32 node_pool_lo = 0, vars_pool_lo = 0;
   // bags + control
   u64* bag[2] = {nullptr, nullptr};
   u32 bag_cap = 0;
@@ -108,8 +109,12 @@ struct hvm_hip_ctx {
     // of batched nets) are outside every wave's slice
     const u32 root_vars = 1 + HVM_HIP_MAX_NETS;
     if (vars_cap <= root_vars) return fail("vars_cap too small for the root var slots");
-    node_slice = (node_cap - 1) / nwaves;
-    vars_slice = (vars_cap - root_vars) / nwaves;
+    // three quarters of each buffer is split into per-wave slices; the rest
+    // is a shared overflow pool for waves whose slice is full
+    node_slice = (u32)(((u64)(node_cap - 1) * 3 / 4) / nwaves);
+    vars_slice = (u32)(((u64)(vars_cap - root_vars) * 3 / 4) / nwaves);
+    node_pool_lo = 1 + nwaves * node_slice;
+    vars_pool_lo = root_vars + nwaves * vars_slice;
     if (node_slice < 4 * HVM_WAVE || vars_slice < 4 * HVM_WAVE) return fail("net too small for the wave count");
     if (!ck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
     if (!ck(hipEventCreate(&ev0), "hipEventCreate") || !ck(hipEventCreate(&ev1), "hipEventCreate")) return false;
@@ -314,6 +319,7 @@ struct hvm_hip_ctx {
     a.node = node; a.vars = vars;
     a.node_cap = node_cap; a.vars_cap = vars_cap;
     a.node_slice = node_slice; a.vars_slice = vars_slice;
+    a.node_pool_lo = node_pool_lo; a.vars_pool_lo = vars_pool_lo;
     a.defs = d_defs; a.pairs = d_pairs; a.ndefs = ndefs;
     a.bag[0] = bag[0]; a.bag[1] = bag[1];
     a.bag_cap = bag_cap;
@@ -323,7 +329,10 @@ struct hvm_hip_ctx {
     a.stats = stats;
     a.root_vars = 1 + HVM_HIP_MAX_NETS;
     a.sched = sched;
-    a.flags = (cfg.flags & HVM_HIP_WAITCNT_FENCE) ? HVM_TURN_WAITCNT_FENCE : 0u;
+    a.flags = (cfg.flags & HVM_HIP_FENCE_RELEASE) ? 0u : HVM_TURN_WAITCNT_FENCE;
+    // a turn ends early once idle_pct% of the waves ran out of work (default 50%)
+    u32 pct = cfg.idle_pct ? (cfg.idle_pct > 100 ? 0u : cfg.idle_pct) : 50u;
+    a.idle_quit = pct ? std::max<u32>(1u, (u32)(((u64)nwaves * pct) / 100)) : 0u;
     return a;
   }
 
@@ -615,7 +624,9 @@ struct hvm_hip_ctx {
     if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return HVM_HIP_E_DEVICE;
     double t0 = now_s();
     deadline = cfg.time_limit_s ? t0 + cfg.time_limit_s : 0;
-    const double budget = 0.5 * (double)node_cap;
+    // admission budget: a quarter of the node capacity, leaving headroom for
+    // the breadth-first growth of parallel evaluation and for the pool
+    const double budget = 0.25 * (double)node_cap;
     double t_norm = 0;
     u32 start = 0;
     while (start < n) {

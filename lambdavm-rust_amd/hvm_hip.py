@@ -25,7 +25,7 @@ KIND_BYTES = {"CALL_COPY": 32, "VOID": 0, "ERAS": 32, "ANNI": 48, "COMM": 112,
               "OPER1": 32, "OPER2": 40, "SWIT0": 32, "SWITN": 40}
 
 NO_READBACK = 1
-WAITCNT_FENCE = 2
+FENCE_RELEASE = 2
 
 # Scheduler (include/hvm_hip.h)
 SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
@@ -45,7 +45,7 @@ class HvmError(RuntimeError):
 class _Cfg(C.Structure):
     _fields_ = [("node_cap", C.c_uint64), ("vars_cap", C.c_uint64), ("bag_cap", C.c_uint32),
                 ("waves", C.c_uint32), ("turn_steps", C.c_uint32), ("flags", C.c_uint32),
-                ("time_limit_s", C.c_uint32)]
+                ("time_limit_s", C.c_uint32), ("idle_pct", C.c_uint32)]
 
 
 class _Out(C.Structure):
@@ -162,8 +162,9 @@ class RunResult:
     diag: dict = field(default_factory=dict)
 
 
-def _cfg(node_cap=0, vars_cap=0, bag_cap=0, waves=0, turn_steps=0, flags=0, time_limit_s=0) -> _Cfg:
-    return _Cfg(node_cap, vars_cap, bag_cap, waves, turn_steps, flags, time_limit_s)
+def _cfg(node_cap=0, vars_cap=0, bag_cap=0, waves=0, turn_steps=0, flags=0, time_limit_s=0,
+         idle_pct=0) -> _Cfg:
+    return _Cfg(node_cap, vars_cap, bag_cap, waves, turn_steps, flags, time_limit_s, idle_pct)
 
 
 def _result(rc: int, o: _Out) -> RunResult:

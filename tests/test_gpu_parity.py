@@ -89,7 +89,10 @@ def test_histogram_matches_oracle(hip, ctx, orc):
 
 
 @pytest.mark.parametrize("cfg", [dict(waves=1), dict(waves=7), dict(turn_steps=1),
-                                 dict(turn_steps=3, waves=64), dict(node_cap=1 << 22, vars_cap=1 << 22)])
+                                 dict(turn_steps=3, waves=64), dict(node_cap=1 << 22, vars_cap=1 << 22),
+                                 dict(idle_pct=101), dict(idle_pct=5), dict(flags=2),
+                                 # small slices: waves overflow into the shared pool
+                                 dict(waves=256, node_cap=1 << 19, vars_cap=1 << 19)])
 def test_schedule_invariance(golden, hip, cfg):
     case = next(c for c in golden["workloads"] if c["name"] == "bitonic_k8_seed3")
     code, expect = P.bitonic(8, 3)

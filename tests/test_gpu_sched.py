@@ -89,7 +89,7 @@ def test_batch_matches_per_net_reference(hip):
     fids = P.def_fids(code)
     entries = [fids[f"net{i}"] for i in range(len(nets))]
     est = [P.batch_estimate(gen, params) for gen, params, _, _ in nets]
-    with hip.Context(time_limit_s=300) as ctx:
+    with hip.Context(time_limit_s=300, node_cap=1 << 29, vars_cap=1 << 29) as ctx:
         ctx.load(hip.compile_book(code))
         per, total = ctx.eval_batch(entries, est)
     assert total.itrs == g["total_itrs"]
