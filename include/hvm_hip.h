@@ -56,6 +56,8 @@ typedef struct hvm_hip_cfg {
  * This flag uses a full agent-scope release fence (with L2 writeback)
  * instead. */
 #define HVM_HIP_FENCE_RELEASE 2u
+/* Time the phases of every evaluator step (reported in phase_ms). */
+#define HVM_HIP_PROFILE 4u
 
 /* Interaction kinds, in the order of hvm_hip_out.kind[]. */
 enum {
@@ -76,6 +78,9 @@ typedef struct hvm_hip_out {
   uint32_t turns;                /* evaluator launches                        */
   uint32_t normalizes;           /* normalize passes (1 + readback expansions) */
   uint32_t err;                  /* evaluator error flags (0 = none)          */
+  double phase_ms[6];            /* with HVM_HIP_PROFILE: wave-time per step phase
+                                    (refill+share, classify, takes+alloc,
+                                    create+release, links+count, unused)     */
   char* result;                  /* "Result:" text as the reference prints it */
   char* canonical;               /* result with canonical var names (tests/run.rs:112-146) */
 } hvm_hip_out;

@@ -50,6 +50,10 @@ struct HvmDevStats {
   unsigned long long defers;      // lane-steps deferred for allocation-table space
   unsigned long long retries;     // interactions re-queued because a node was not yet visible
   unsigned long long probes;      // 64-slot allocator windows scanned
+  // wall-clock ticks (100 MHz) spent per step phase, summed over waves, when
+  // the evaluator runs with HVM_TURN_PROFILE: 0 refill+share, 1 classify+
+  // reserve, 2 takes+allocation, 3 creation+release, 4 links, 5 counting
+  unsigned long long phase[6];
 };
 
 // Error flags (bitwise) reported by the evaluator.
@@ -151,6 +155,8 @@ struct HvmTurnArgs {
 // Release before the links: the default is the agent-scope release fence;
 // this flag replaces it by a wait for the completion of the creation stores.
 #define HVM_TURN_WAITCNT_FENCE 1u
+// Accumulate per-phase wall-clock time into HvmDevStats::phase.
+#define HVM_TURN_PROFILE 2u
 
 struct HvmGatherArgs {
   const unsigned long long* node;

@@ -13,7 +13,7 @@
 //   P2 wave-cooperative node/var allocation     replaces src/hvm.c:713-780
 //   P3 node takes (64-bit atomic exchange)      src/hvm.c:909-1069
 //   P4 node/var creation (device-scope stores)
-//   P5 one device-scope release fence per wave
+//   P5 release: wait for the creation stores (or an agent release fence)
 //   P6 links (32-bit exchanges on vars)         src/hvm.c:784-846
 //   P7 counting (ITRS += 1 per non-LINK rule)   src/hvm.c:1116-1118
 // Between steps, idle lanes take the oldest redex of a lane holding two or
@@ -153,17 +153,73 @@ __device__ __forceinline__ u32 enter(const HvmTurnArgs& A, u32 var) {
   return var;
 }
 
-__device__ __forceinline__ void link(WaveLds& L, Lane& ln, const HvmTurnArgs& A, u32 a, u32 b) {
-  for (u32 it = 0; it < HVM_LINK_CAP; ++it) {
-    if (hvm_get_tag(a) != HVM_VAR && hvm_get_tag(b) == HVM_VAR) { u32 x = a; a = b; b = x; }
-    if (hvm_get_tag(a) != HVM_VAR) { push_redex(L, ln, A, hvm_new_pair(a, b)); return; }
-    b = enter(A, b);
-    u32 a_ = vars_xchg(A, hvm_get_val(a), b);
-    if (a_ == HVM_NONE) return;
-    vars_put(A, hvm_get_val(a), 0);
-    a = a_;
+// Links up to HVM_LB pairs of one interaction together.  Each pair runs the
+// reference's link loop (src/hvm.c:812-842) as a small state machine; every
+// round, each unfinished pair issues its next exchange, and all of a lane's
+// exchanges of the round are in flight at once.  Interleaving the links of one
+// interaction is the same as running them on different threads, which the
+// lock-free linker allows.  A link ends either by publishing its var or with
+// one new redex; the redexes are pushed in pair order once every pair is done,
+// so the deque receives them in the order a one-at-a-time linker would push
+// them (the deque is LIFO, and that order keeps the evaluation depth-first).
+#define HVM_LB 4
+enum { LK_DONE = 0, LK_ENTER = 1, LK_PUBLISH = 2, LK_ORIENT = 3 };
+
+__device__ __forceinline__ void link_batch(WaveLds& L, Lane& ln, const HvmTurnArgs& A, u32* pa, u32* pb, u32 n) {
+  u32 st[HVM_LB];
+  u32 got = 0;          // pairs that ended with a redex (pa[i] ~ pb[i])
+  bool busy = true;     // some pair is not done
+#pragma unroll
+  for (int i = 0; i < HVM_LB; ++i) st[i] = (u32)i < n ? (u32)LK_ORIENT : (u32)LK_DONE;
+  for (u32 round = 0; round < HVM_LINK_CAP; ++round) {
+#pragma unroll
+    for (int i = 0; i < HVM_LB; ++i) {
+      if (st[i] == LK_ORIENT) {
+        // A points to B: make A the var; two non-vars are a new redex
+        if (hvm_get_tag(pa[i]) != HVM_VAR && hvm_get_tag(pb[i]) == HVM_VAR) { u32 x = pa[i]; pa[i] = pb[i]; pb[i] = x; }
+        if (hvm_get_tag(pa[i]) != HVM_VAR) {
+          st[i] = LK_DONE;
+          got |= 1u << i;
+        } else {
+          st[i] = hvm_get_tag(pb[i]) == HVM_VAR ? (u32)LK_ENTER : (u32)LK_PUBLISH;
+        }
+      }
+    }
+    u32 r[HVM_LB];
+#pragma unroll
+    for (int i = 0; i < HVM_LB; ++i) {
+      r[i] = 0;
+      if (st[i] == LK_ENTER) r[i] = vars_xchg(A, hvm_get_val(pb[i]), HVM_NONE);   // extend B
+      else if (st[i] == LK_PUBLISH) r[i] = vars_xchg(A, hvm_get_val(pa[i]), pb[i]);  // A ~> B
+    }
+    busy = false;
+#pragma unroll
+    for (int i = 0; i < HVM_LB; ++i) {
+      if (st[i] == LK_ENTER) {
+        if (r[i] == HVM_NONE || r[i] == 0) {
+          st[i] = LK_PUBLISH;                       // B is an unbound var
+        } else {
+          vars_put(A, hvm_get_val(pb[i]), 0);       // we own both: free B
+          pb[i] = r[i];
+          st[i] = hvm_get_tag(pb[i]) == HVM_VAR ? (u32)LK_ENTER : (u32)LK_PUBLISH;
+        }
+      } else if (st[i] == LK_PUBLISH) {
+        if (r[i] == HVM_NONE) {
+          st[i] = LK_DONE;                          // the other side will find B
+        } else {
+          vars_put(A, hvm_get_val(pa[i]), 0);       // A was bound: free it, link A' ~ B
+          pa[i] = r[i];
+          st[i] = LK_ORIENT;
+        }
+      }
+      busy |= st[i] != LK_DONE;
+    }
+    if (!busy) break;
   }
-  atomicOr(&A.ctl->err, (u32)HVM_ERR_LINK_LOOP);
+  if (__builtin_expect(busy, 0)) atomicOr(&A.ctl->err, (u32)HVM_ERR_LINK_LOOP);
+#pragma unroll
+  for (int i = 0; i < HVM_LB; ++i)
+    if (got & (1u << i)) push_redex(L, ln, A, hvm_new_pair(pa[i], pb[i]));
 }
 
 // ---------------------------------------------------------------------------
@@ -309,7 +365,19 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
   }
   bool more = n_in > HVM_WAVE * A.nwaves;
 
+  // optional per-phase timing (HVM_TURN_PROFILE), wall clock at 100 MHz
+  const bool prof = (A.flags & HVM_TURN_PROFILE) != 0;
+  u64 ph[6] = {0, 0, 0, 0, 0, 0};
+  u64 t_prev = 0;
+#define HVM_PHASE(k)                        \
+  if (prof) {                               \
+    u64 t_now = wall_clock64();             \
+    ph[k] += t_now - t_prev;                \
+    t_prev = t_now;                         \
+  }
+
   for (u32 step = 0; step < max_steps; ++step) {
+    if (prof) t_prev = wall_clock64();
     // ---- refill from the input bag ----
     if (more) {
       bool empty = ln.top == ln.bot;
@@ -362,6 +430,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     }
     steps++;
     lane_steps += __popcll(__ballot(has));
+    HVM_PHASE(0);
 
     // ---- P0: peek + classify ----
     u64 rx = has ? deq_at(L, ln.top - 1) : 0ull;
@@ -397,6 +466,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
     u32 need_v = wave_max(run ? off_v + nv : 0);
     if (run) ln.top--;
 
+    HVM_PHASE(1);
     // ---- P3: takes, issued before the allocator's probes so that both
     //      round trips overlap ----
     bool take_a = run && (rule == HVM_ANNI || rule == HVM_COMM);
@@ -441,6 +511,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       }
     }
 
+    HVM_PHASE(2);
     // ---- P4: creation ----
     const u32* tn = L.tbl_n + off_n;
     const u32* tv = L.tbl_v + off_v;
@@ -487,56 +558,64 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     }
 
+    HVM_PHASE(3);
     // ---- P6: links ----
     if (run) {
+      u32 pa[HVM_LB], pb[HVM_LB], np = 0;
       switch (rule) {
         case HVM_LINK:
-          link(L, ln, A, a, b);
+          pa[0] = a; pb[0] = b; np = 1;
           break;
         case HVM_CALL:
           if (copy) {
-            link(L, ln, A, a, B1);
-            link(L, ln, A, a, B2);
+            pa[0] = a; pb[0] = B1; pa[1] = a; pb[1] = B2; np = 2;
           } else {
-            for (u32 i = 0; i < d.rlen; ++i) {
-              u64 p = A.pairs[d.rbag_off + i];
-              u32 p1 = hvm_get_fst(p), p2 = hvm_get_snd(p);
+            // the def's redexes, then its root against the caller's port,
+            // HVM_LB links at a time
+            for (u32 i = 0; i <= d.rlen; ++i) {
+              u32 p1, p2;
+              if (i < d.rlen) {
+                u64 p = A.pairs[d.rbag_off + i];
+                p1 = hvm_get_fst(p); p2 = hvm_get_snd(p);
+              } else {
+                p1 = d.root; p2 = b;
+              }
               if (hvm_is_nod(p1)) p1 = hvm_new_port(hvm_get_tag(p1), tn[hvm_get_val(p1)]);
               else if (hvm_is_var(p1)) p1 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p1)]);
-              if (hvm_is_nod(p2)) p2 = hvm_new_port(hvm_get_tag(p2), tn[hvm_get_val(p2)]);
-              else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
-              link(L, ln, A, p1, p2);
+              if (i < d.rlen) {
+                if (hvm_is_nod(p2)) p2 = hvm_new_port(hvm_get_tag(p2), tn[hvm_get_val(p2)]);
+                else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
+              }
+              pa[np] = p1; pb[np] = p2; np++;
+              if (np == HVM_LB && i < d.rlen) { link_batch(L, ln, A, pa, pb, np); np = 0; }
             }
-            u32 r = d.root;
-            if (hvm_is_nod(r)) r = hvm_new_port(hvm_get_tag(r), tn[hvm_get_val(r)]);
-            else if (hvm_is_var(r)) r = hvm_new_port(HVM_VAR, tv[hvm_get_val(r)]);
-            link(L, ln, A, r, b);
           }
           break;
         case HVM_ERAS:
-          link(L, ln, A, a, B1);
-          link(L, ln, A, a, B2);
+          pa[0] = a; pb[0] = B1; pa[1] = a; pb[1] = B2; np = 2;
           break;
         case HVM_ANNI:
-          link(L, ln, A, hvm_get_fst(NA), B1);
-          link(L, ln, A, hvm_get_snd(NA), B2);
+          pa[0] = hvm_get_fst(NA); pb[0] = B1; pa[1] = hvm_get_snd(NA); pb[1] = B2; np = 2;
           break;
         case HVM_COMM:
-          link(L, ln, A, hvm_new_port(hvm_get_tag(b), tn[0]), hvm_get_fst(NA));
-          link(L, ln, A, hvm_new_port(hvm_get_tag(b), tn[1]), hvm_get_snd(NA));
-          link(L, ln, A, hvm_new_port(hvm_get_tag(a), tn[2]), B1);
-          link(L, ln, A, hvm_new_port(hvm_get_tag(a), tn[3]), B2);
+          pa[0] = hvm_new_port(hvm_get_tag(b), tn[0]); pb[0] = hvm_get_fst(NA);
+          pa[1] = hvm_new_port(hvm_get_tag(b), tn[1]); pb[1] = hvm_get_snd(NA);
+          pa[2] = hvm_new_port(hvm_get_tag(a), tn[2]); pb[2] = B1;
+          pa[3] = hvm_new_port(hvm_get_tag(a), tn[3]); pb[3] = B2;
+          np = 4;
           break;
         case HVM_OPER:
-          if (oper1) link(L, ln, A, hvm_new_port(HVM_NUM, cv), B2);
-          else link(L, ln, A, B1, hvm_new_port(HVM_OPR, tn[0]));
+          if (oper1) { pa[0] = hvm_new_port(HVM_NUM, cv); pb[0] = B2; }
+          else { pa[0] = B1; pb[0] = hvm_new_port(HVM_OPR, tn[0]); }
+          np = 1;
           break;
         case HVM_SWIT:
-          link(L, ln, A, hvm_new_port(HVM_CON, tn[0]), B1);
+          pa[0] = hvm_new_port(HVM_CON, tn[0]); pb[0] = B1; np = 1;
           break;
         default:
           break;
       }
+      if (np) link_batch(L, ln, A, pa, pb, np);
 
       // ---- P7: counting ----
       switch (rule) {
@@ -559,7 +638,9 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
         default: break;
       }
     }
+    HVM_PHASE(4);
   }
+#undef HVM_PHASE
 
   // ---- end of turn: everything left goes to the output bag ----
   {
@@ -616,6 +697,10 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
       if (defers) atomicAdd(&A.stats->defers, defers);
       if (retries) atomicAdd(&A.stats->retries, retries);
       if (probes) atomicAdd(&A.stats->probes, probes);
+      if (prof) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) if (ph[k]) atomicAdd(&A.stats->phase[k], ph[k]);
+      }
       A.wave_state[w] = ws;
       // per-turn counters for the scheduler
       HvmTurnStats* T = &A.sched->turn;

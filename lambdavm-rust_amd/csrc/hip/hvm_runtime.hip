@@ -330,6 +330,7 @@ struct hvm_hip_ctx {
     a.root_vars = 1 + HVM_HIP_MAX_NETS;
     a.sched = sched;
     a.flags = (cfg.flags & HVM_HIP_FENCE_RELEASE) ? 0u : HVM_TURN_WAITCNT_FENCE;
+    if (cfg.flags & HVM_HIP_PROFILE) a.flags |= HVM_TURN_PROFILE;
     // a turn ends early once idle_pct% of the waves ran out of work (default 50%)
     u32 pct = cfg.idle_pct ? (cfg.idle_pct > 100 ? 0u : cfg.idle_pct) : 50u;
     a.idle_quit = pct ? std::max<u32>(1u, (u32)(((u64)nwaves * pct) / 100)) : 0u;
@@ -545,6 +546,7 @@ struct hvm_hip_ctx {
       out->bytes = s.bytes;
       out->steps = s.steps; out->spills = s.spills; out->defers = s.defers;
       out->retries = s.retries; out->probes = s.probes;
+      for (int k = 0; k < 6; ++k) out->phase_ms[k] = (double)s.phase[k] / (double)clock_khz;
       out->kernel_ms = kernel_ms;
       out->eval_ms = (t_eval - t0) * 1e3;
       out->turns = turns;
@@ -604,6 +606,7 @@ struct hvm_hip_ctx {
     out->bytes = s.bytes;
     out->steps = s.steps; out->spills = s.spills; out->defers = s.defers;
     out->retries = s.retries; out->probes = s.probes;
+    for (int k = 0; k < 6; ++k) out->phase_ms[k] = (double)s.phase[k] / (double)clock_khz;
     out->kernel_ms = kernel_ms;
     out->eval_ms = (t_eval - t0) * 1e3;
     out->turns = turns;

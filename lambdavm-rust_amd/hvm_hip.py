@@ -26,6 +26,7 @@ KIND_BYTES = {"CALL_COPY": 32, "VOID": 0, "ERAS": 32, "ANNI": 48, "COMM": 112,
 
 NO_READBACK = 1
 FENCE_RELEASE = 2
+PROFILE = 4
 
 # Scheduler (include/hvm_hip.h)
 SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
@@ -54,6 +55,7 @@ class _Out(C.Structure):
                 ("defers", C.c_uint64), ("retries", C.c_uint64), ("probes", C.c_uint64),
                 ("kernel_ms", C.c_double), ("eval_ms", C.c_double), ("wall_s", C.c_double),
                 ("turns", C.c_uint32), ("normalizes", C.c_uint32), ("err", C.c_uint32),
+                ("phase_ms", C.c_double * 6),
                 ("result", C.c_void_p), ("canonical", C.c_void_p)]
 
 
@@ -172,7 +174,8 @@ def _result(rc: int, o: _Out) -> RunResult:
                   bytes=o.bytes, leak=o.leak, kernel_ms=o.kernel_ms, eval_ms=o.eval_ms,
                   wall_s=o.wall_s, turns=o.turns, normalizes=o.normalizes, err=o.err,
                   diag={"steps": o.steps, "spills": o.spills, "defers": o.defers,
-                        "retries": o.retries, "probes": o.probes})
+                        "retries": o.retries, "probes": o.probes,
+                        "phase_ms": [round(o.phase_ms[k], 3) for k in range(5)]})
     r.result = _take_str(o.result)
     r.canonical = _take_str(o.canonical)
     return r
