@@ -48,12 +48,38 @@ def cpu_baseline(words, itrs_expected):
             "sample": "one full run of the bench workload by the single-thread C restatement (oracle/)"}
 
 
+def measured_traffic(k):
+    """HBM bytes per turn-kernel dispatch for this workload, from the committed
+    rocprofv3 counter summary: FETCH_SIZE and WRITE_SIZE (KiB, summed over the
+    profiled dispatches), each scaled by the factor the same counter needs to
+    read a plain device copy of known size correctly.  None if absent."""
+    path = os.path.join(ROOT, "profiles", f"hvm_turn_kernel.bitonic{k}.md")
+    if not os.path.exists(path):
+        return None, None
+    vals = {}
+    for line in open(path):
+        parts = line.strip().strip("`").split(":")
+        if len(parts) == 2 and parts[0].strip().isidentifier():
+            try:
+                vals[parts[0].strip()] = float(parts[1])
+            except ValueError:
+                pass
+    need = ("FETCH_SIZE_KiB", "WRITE_SIZE_KiB", "fetch_copy_factor", "write_copy_factor", "turn_dispatches")
+    if not all(n in vals for n in need):
+        return None, None
+    b = (vals["FETCH_SIZE_KiB"] * vals["fetch_copy_factor"] +
+         vals["WRITE_SIZE_KiB"] * vals["write_copy_factor"]) * 1024 / vals["turn_dispatches"]
+    return round(b, 1), os.path.relpath(path, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--k", type=int, default=16, help="log2 of the key count")
+    ap.add_argument("--sched", choices=("plain", "learned"), default="plain",
+                    help="turn-length scheduler of the timed steps (the other one is timed after them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (torch), to calibrate "
@@ -93,8 +119,22 @@ def main():
               "sorted_keys_match": P.flatten_tree(P.parse_tree_text(r0.result)) == sorted_keys}
     ctx.close()
 
+    weights = None
+    wpath = os.path.join(ROOT, "lambdavm-rust_amd", "sched_weights.json")
+    if os.path.exists(wpath):
+        weights = json.load(open(wpath))["params_fp16"]
+    if args.sched == "learned" and weights is None:
+        sys.exit("bench.py: --sched learned needs lambdavm-rust_amd/sched_weights.json")
+
+    def use(ctx, sched):
+        if sched == "learned":
+            ctx.set_scheduler(hvm_hip.SCHED_LEARNED, params_fp16=weights)
+        else:
+            ctx.set_scheduler(hvm_hip.SCHED_PLAIN)
+
     ctx = hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK)
     ctx.load(words)
+    use(ctx, args.sched)
     for _ in range(max(0, args.warmup - 1)):
         ctx.eval()
     runs = []
@@ -102,6 +142,17 @@ def main():
     for _ in range(args.steps):
         runs.append(ctx.eval())
     t1 = time.perf_counter()
+    # the other scheduler on the same workload, after the timed region
+    other = "plain" if args.sched == "learned" else "learned"
+    compare = {args.sched + "_mips": None, other + "_mips": None}
+    if weights is not None:
+        use(ctx, other)
+        ctx.eval()
+        t2 = time.perf_counter()
+        o_runs = [ctx.eval() for _ in range(args.steps)]
+        t3 = time.perf_counter()
+        assert all(r.itrs == runs[0].itrs for r in o_runs), "ITRS changed with the scheduler"
+        compare[other + "_mips"] = round(runs[0].itrs * args.steps / (t3 - t2) / 1e6, 2)
     ctx.close()
 
     itrs = runs[0].itrs
@@ -110,11 +161,15 @@ def main():
     ms_per_step = total_s / args.steps * 1e3
     value = itrs * args.steps / total_s / 1e6
     kernel_ms = sum(r.kernel_ms for r in runs)
-    turns = sum(r.turns for r in runs)
+    # every dispatch of the turn kernel, including those that found the net
+    # already normal, as rocprofv3 lists them
+    launches = sum(r.launches for r in runs)
+    launch_ms = sum(r.launch_ms for r in runs)
     bytes_total = sum(r.bytes for r in runs)
-    avg_launch_ms = kernel_ms / turns
-    bytes_per_launch = bytes_total / turns
+    avg_launch_ms = launch_ms / launches
+    bytes_per_launch = bytes_total / launches
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args.k)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -129,16 +184,19 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": f"bitonic sort of 2^{args.k} u24 keys (SplitMix64 seed 0), depth-{args.k} key tree",
-                   "keys": 1 << args.k, "itrs_per_step": itrs, "scheduler": "plain"},
+                   "keys": 1 << args.k, "itrs_per_step": itrs, "scheduler": args.sched},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "hvm_turn_kernel", "launches_per_step": turns / args.steps,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": "hvm_turn_kernel", "launches_per_step": launches / args.steps,
                      "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
                      "avg_launch_ms": round(avg_launch_ms, 4)},
         "kernel_mips": round(itrs * args.steps / (kernel_ms * 1e-3) / 1e6, 2),
         "eval_ms_per_step": round(sum(r.eval_ms for r in runs) / args.steps, 3),
         "parity": parity,
+        "schedulers": compare,
     }
+    compare[args.sched + "_mips"] = out["value"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(words, itrs)
     print(json.dumps(out), flush=True)

@@ -75,9 +75,12 @@ typedef struct hvm_hip_out {
   double kernel_ms;              /* sum of evaluator kernel durations         */
   double eval_ms;                /* boot → normal form, incl. readback expansions */
   double wall_s;                 /* boot → result read back (reference window) */
-  uint32_t turns;                /* evaluator launches                        */
+  uint32_t turns;                /* evaluator turns that ran                  */
   uint32_t normalizes;           /* normalize passes (1 + readback expansions) */
   uint32_t err;                  /* evaluator error flags (0 = none)          */
+  uint32_t launches;             /* turn-kernel dispatches, including those that
+                                    found the net already normal and returned */
+  double launch_ms;              /* summed duration of all `launches`         */
   double phase_ms[6];            /* with HVM_HIP_PROFILE: wave-time per step phase
                                     (refill+share, classify, takes+alloc,
                                     create+release, links+count, unused)     */

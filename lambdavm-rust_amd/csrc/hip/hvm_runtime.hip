@@ -71,8 +71,8 @@ struct hvm_hip_ctx {
   u32 ndefs = 0;
   bool loaded = false;
   // per-eval bookkeeping
-  double kernel_ms = 0;
-  u32 turns = 0, normalizes = 0;
+  double kernel_ms = 0, launch_ms = 0;
+  u32 turns = 0, normalizes = 0, launches = 0;
   std::string last_error;
 
   bool fail(const std::string& m) { last_error = m; return false; }
@@ -367,11 +367,13 @@ struct hvm_hip_ctx {
       if (!ck(hipMemcpyAsync(&host_ctl, ctl, sizeof(HvmCtl), hipMemcpyDeviceToHost, stream), "ctl readback")) return false;
       if (!ck(hipStreamSynchronize(stream), "turn sync")) return false;
       u32 ran = host_ctl.turns - turns_before;
-      for (u32 i = 0; i < chunk && i < ran; ++i) {
+      for (u32 i = 0; i < chunk; ++i) {
         float ms = 0;
         hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]);
-        kernel_ms += ms;
+        if (i < ran) kernel_ms += ms;
+        launch_ms += ms;
       }
+      launches += chunk;
       turns += ran;
       turns_before = host_ctl.turns;
       if (host_ctl.err) return fail("evaluator error flags 0x" + std::to_string(host_ctl.err));
@@ -527,7 +529,9 @@ struct hvm_hip_ctx {
   int eval(hvm_hip_out* out) {
     if (!loaded) { fail("no book loaded"); return HVM_HIP_E_ARG; }
     kernel_ms = 0;
+    launch_ms = 0;
     turns = 0;
+    launches = 0;
     normalizes = 0;
     eval_started = false;
     if (!reset_net()) return HVM_HIP_E_DEVICE;
@@ -550,6 +554,8 @@ struct hvm_hip_ctx {
       out->kernel_ms = kernel_ms;
       out->eval_ms = (t_eval - t0) * 1e3;
       out->turns = turns;
+      out->launches = launches;
+      out->launch_ms = launch_ms;
       out->normalizes = normalizes;
     }
     if (rc != 0) {
@@ -610,6 +616,8 @@ struct hvm_hip_ctx {
     out->kernel_ms = kernel_ms;
     out->eval_ms = (t_eval - t0) * 1e3;
     out->turns = turns;
+    out->launches = launches;
+    out->launch_ms = launch_ms;
     out->normalizes = normalizes;
   }
 
@@ -621,7 +629,9 @@ struct hvm_hip_ctx {
       memset(&outs[j], 0, sizeof outs[j]);
     }
     kernel_ms = 0;
+    launch_ms = 0;
     turns = 0;
+    launches = 0;
     normalizes = 0;
     eval_started = false;
     if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return HVM_HIP_E_DEVICE;

@@ -55,6 +55,7 @@ class _Out(C.Structure):
                 ("defers", C.c_uint64), ("retries", C.c_uint64), ("probes", C.c_uint64),
                 ("kernel_ms", C.c_double), ("eval_ms", C.c_double), ("wall_s", C.c_double),
                 ("turns", C.c_uint32), ("normalizes", C.c_uint32), ("err", C.c_uint32),
+                ("launches", C.c_uint32), ("launch_ms", C.c_double),
                 ("phase_ms", C.c_double * 6),
                 ("result", C.c_void_p), ("canonical", C.c_void_p)]
 
@@ -159,6 +160,8 @@ class RunResult:
     turns: int = 0
     normalizes: int = 0
     err: int = 0
+    launches: int = 0
+    launch_ms: float = 0.0
     result: Optional[str] = None
     canonical: Optional[str] = None
     diag: dict = field(default_factory=dict)
@@ -173,6 +176,7 @@ def _result(rc: int, o: _Out) -> RunResult:
     r = RunResult(status=rc, itrs=o.itrs, kinds={k: o.kind[i] for i, k in enumerate(KINDS)},
                   bytes=o.bytes, leak=o.leak, kernel_ms=o.kernel_ms, eval_ms=o.eval_ms,
                   wall_s=o.wall_s, turns=o.turns, normalizes=o.normalizes, err=o.err,
+                  launches=o.launches, launch_ms=o.launch_ms,
                   diag={"steps": o.steps, "spills": o.spills, "defers": o.defers,
                         "retries": o.retries, "probes": o.probes,
                         "phase_ms": [round(o.phase_ms[k], 3) for k in range(5)]})

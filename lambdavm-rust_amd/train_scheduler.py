@@ -51,8 +51,9 @@ def main():
     with H.Context(time_limit_s=300) as ctx:
         ctx.load(words)
         for i in range(args.eval_runs):
+            seed = args.train_seeds + i  # held out: no trace run used it
             for mode in ((H.SCHED_PLAIN, H.SCHED_LEARNED) if i % 2 == 0 else (H.SCHED_LEARNED, H.SCHED_PLAIN)):
-                ctx.set_scheduler(mode, params_fp16=p16 if mode == H.SCHED_LEARNED else None)
+                ctx.set_scheduler(mode, seed=seed, params_fp16=p16 if mode == H.SCHED_LEARNED else None)
                 r = ctx.eval()
                 (plain if mode == H.SCHED_PLAIN else learned).append(r.eval_ms)
                 results.add((r.itrs, r.canonical))
@@ -65,6 +66,7 @@ def main():
         "speedup": statistics.median(plain) / statistics.median(learned),
         "identical_results": len(results) == 1,
         "eval_runs": args.eval_runs,
+        "eval_seeds": [args.train_seeds, args.train_seeds + args.eval_runs - 1],
     }
     print(json.dumps(report), flush=True)
     with open(args.out, "w") as f:

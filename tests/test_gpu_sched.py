@@ -96,3 +96,29 @@ def test_batch_matches_per_net_reference(hip):
     for i, (row, r) in enumerate(zip(g["nets"], per)):
         assert r.err == 0, i
         assert hashlib.sha256(r.canonical.encode()).hexdigest() == row["nf_sha256"], i
+
+
+def test_trained_scheduler_gain(hip):
+    """The committed scheduler weights (trained on church(12, 12) traces from
+    seeds 0-63) against the plain heuristic at held-out seeds: identical
+    normal form and ITRS in every run, and the learned schedule is not slower.
+    Prints the measured gain."""
+    path = os.path.join(ROOT, "lambdavm-rust_amd", "sched_weights.json")
+    w = json.load(open(path))
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+    case = next(c for c in g["workloads"] if c["generator"] == "church" and c["params"] == {"n": 12, "m": 12})
+    code, expect = P.church(12, 12)
+    ms = {hip.SCHED_PLAIN: [], hip.SCHED_LEARNED: []}
+    with hip.Context(time_limit_s=120) as ctx:
+        ctx.load(hip.compile_book(code))
+        ctx.eval()
+        for i, seed in enumerate(range(80, 88)):
+            order = (hip.SCHED_PLAIN, hip.SCHED_LEARNED) if i % 2 == 0 else (hip.SCHED_LEARNED, hip.SCHED_PLAIN)
+            for mode in order:
+                ctx.set_scheduler(mode, seed=seed, params_fp16=w["params_fp16"] if mode == hip.SCHED_LEARNED else None)
+                r = ctx.eval()
+                assert r.itrs == case["itrs"] and r.result == str(expect)
+                ms[mode].append(r.eval_ms)
+    plain, learned = np.median(ms[hip.SCHED_PLAIN]), np.median(ms[hip.SCHED_LEARNED])
+    print(f"church(12,12) median eval: plain {plain:.2f} ms, learned {learned:.2f} ms, gain {plain / learned:.3f}x")
+    assert learned <= plain * 1.05
