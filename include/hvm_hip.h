@@ -81,6 +81,7 @@ typedef struct hvm_hip_out {
   uint32_t launches;             /* turn-kernel dispatches, including those that
                                     found the net already normal and returned */
   double launch_ms;              /* summed duration of all `launches`         */
+  uint32_t waves;                /* evaluator waves (workgroups) per turn     */
   double phase_ms[6];            /* with HVM_HIP_PROFILE: wave-time per step phase
                                     (refill+share, classify, takes+alloc,
                                     create+release, links+count, unused)     */

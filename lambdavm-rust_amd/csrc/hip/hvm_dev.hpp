@@ -10,7 +10,13 @@
 // ---------------------------------------------------------------------------
 
 #define HVM_WAVE        64     // gfx950 wavefront width
+#ifndef HVM_DEQ
 #define HVM_DEQ         32     // per-lane redex deque entries (power of two)
+#endif
+// minimum resident waves per SIMD the turn kernel is compiled for
+#ifndef HVM_TURN_OCC
+#define HVM_TURN_OCC    1
+#endif
 #define HVM_TBL_NODE    512    // per-wave node-slot table (one step's allocations)
 #define HVM_TBL_VARS    512    // per-wave var-slot table
 

@@ -324,7 +324,7 @@ __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u3
 // The turn kernel
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(HVM_WAVE) void hvm_turn_kernel(HvmTurnArgs A) {
+__global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTurnArgs A) {
   __shared__ WaveLds L;
   const u32 lane = lane_id();
   const u32 w = blockIdx.x;
@@ -928,3 +928,11 @@ extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s) {
 }
 
 extern "C" size_t hvm_turn_lds_bytes(void) { return sizeof(WaveLds); }
+
+// Turn-kernel workgroups (one wave each) that can be resident on one CU at
+// once, given the kernel's registers and LDS; 0 if the runtime cannot tell.
+extern "C" int hvm_turn_waves_per_cu(void) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, hvm_turn_kernel, HVM_WAVE, 0) != hipSuccess) return 0;
+  return n;
+}

@@ -21,6 +21,7 @@ extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, uint32_t clock_kh
 extern "C" uint32_t hvm_plain_action_host(uint32_t n, uint32_t nwaves);
 extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves);
 extern "C" size_t hvm_turn_lds_bytes(void);
+extern "C" int hvm_turn_waves_per_cu(void);
 
 namespace {
 
@@ -98,9 +99,11 @@ struct hvm_hip_ctx {
     if (cfg.waves) {
       nwaves = cfg.waves;
     } else {
-      // Fill the chip: as many one-wave workgroups as the LDS allows per CU.
+      // Fill the chip: as many one-wave workgroups as can be resident per CU
+      // (registers and LDS), so that every wave of a turn runs at once.
+      int occ = hvm_turn_waves_per_cu();
       size_t lds = hvm_turn_lds_bytes();
-      size_t per_cu = (160u * 1024u) / (lds ? lds : 1);
+      size_t per_cu = occ > 0 ? (size_t)occ : (160u * 1024u) / (lds ? lds : 1);
       if (per_cu < 1) per_cu = 1;
       if (per_cu > 16) per_cu = 16;
       nwaves = (u32)(cus * per_cu);
@@ -556,6 +559,7 @@ struct hvm_hip_ctx {
       out->turns = turns;
       out->launches = launches;
       out->launch_ms = launch_ms;
+      out->waves = nwaves;
       out->normalizes = normalizes;
     }
     if (rc != 0) {
@@ -618,6 +622,7 @@ struct hvm_hip_ctx {
     out->turns = turns;
     out->launches = launches;
     out->launch_ms = launch_ms;
+    out->waves = nwaves;
     out->normalizes = normalizes;
   }
 

@@ -55,7 +55,7 @@ class _Out(C.Structure):
                 ("defers", C.c_uint64), ("retries", C.c_uint64), ("probes", C.c_uint64),
                 ("kernel_ms", C.c_double), ("eval_ms", C.c_double), ("wall_s", C.c_double),
                 ("turns", C.c_uint32), ("normalizes", C.c_uint32), ("err", C.c_uint32),
-                ("launches", C.c_uint32), ("launch_ms", C.c_double),
+                ("launches", C.c_uint32), ("launch_ms", C.c_double), ("waves", C.c_uint32),
                 ("phase_ms", C.c_double * 6),
                 ("result", C.c_void_p), ("canonical", C.c_void_p)]
 
@@ -178,7 +178,7 @@ def _result(rc: int, o: _Out) -> RunResult:
                   wall_s=o.wall_s, turns=o.turns, normalizes=o.normalizes, err=o.err,
                   launches=o.launches, launch_ms=o.launch_ms,
                   diag={"steps": o.steps, "spills": o.spills, "defers": o.defers,
-                        "retries": o.retries, "probes": o.probes,
+                        "retries": o.retries, "probes": o.probes, "waves": o.waves,
                         "phase_ms": [round(o.phase_ms[k], 3) for k in range(5)]})
     r.result = _take_str(o.result)
     r.canonical = _take_str(o.canonical)
