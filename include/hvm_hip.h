@@ -58,6 +58,12 @@ typedef struct hvm_hip_cfg {
 #define HVM_HIP_FENCE_RELEASE 2u
 /* Time the phases of every evaluator step (reported in phase_ms). */
 #define HVM_HIP_PROFILE 4u
+/* Local-net evaluator: every wave allocates nodes and vars in its own LDS net
+ * and copies a node to HBM only when an HBM var must point to it (a LEAK,
+ * counted in `leak`; src/hvm.cu:1129-1187); at the end of a turn each wave
+ * moves what its local net still holds to HBM (src/hvm.cu:1426-1482).  Net
+ * capacities are limited to 2^28 slots with it. */
+#define HVM_HIP_LOCAL_NET 8u
 
 /* Interaction kinds, in the order of hvm_hip_out.kind[]. */
 enum {
@@ -70,7 +76,8 @@ typedef struct hvm_hip_out {
   uint64_t itrs;                 /* ITRS, counted as src/hvm.c:1116-1118      */
   uint64_t kind[HVM_HIP_KINDS];  /* per-kind interaction counts               */
   uint64_t bytes;                /* algorithmic bytes of all interactions     */
-  uint64_t leak;                 /* LEAK line value (no local nets: 0)        */
+  uint64_t leak;                 /* LEAK line value: LEAK interactions (0
+                                    without HVM_HIP_LOCAL_NET)                */
   uint64_t steps, spills, defers, retries, probes;  /* scheduler diagnostics  */
   double kernel_ms;              /* sum of evaluator kernel durations         */
   double eval_ms;                /* boot → normal form, incl. readback expansions */
@@ -82,6 +89,9 @@ typedef struct hvm_hip_out {
                                     found the net already normal and returned */
   double launch_ms;              /* summed duration of all `launches`         */
   uint32_t waves;                /* evaluator waves (workgroups) per turn     */
+  uint64_t saves;                /* HVM_HIP_LOCAL_NET: redexes moved to HBM at turn ends */
+  uint64_t local_left;           /* HVM_HIP_LOCAL_NET: local slots left in use after
+                                    turn ends (unreachable garbage)           */
   double phase_ms[6];            /* with HVM_HIP_PROFILE: wave-time per step phase
                                     (refill+share, classify, takes+alloc,
                                     create+release, links+count, unused)     */

@@ -19,6 +19,23 @@
 #endif
 #define HVM_TBL_NODE    512    // per-wave node-slot table (one step's allocations)
 #define HVM_TBL_VARS    512    // per-wave var-slot table
+#define HVM_TBL_X       128    // more entries of each table: HBM slots of the
+                               // local-net evaluator's LEAK and SAVE steps
+
+// Local net (the local-net evaluator): each wave owns HVM_LNODE node slots and
+// HVM_LVARS var slots in LDS.  A node or var port whose value is
+// HVM_LOCAL_BASE + i (and not ROOT) addresses slot i of the owning wave's
+// local net; HBM slots are below HVM_LOCAL_BASE, so the net's capacities are
+// at most 2^28 slots with it.
+#ifndef HVM_LNODE
+#define HVM_LNODE       1024
+#endif
+#ifndef HVM_LVARS
+#define HVM_LVARS       1024
+#endif
+#define HVM_LOCAL_BASE  0x10000000u
+// per-wave HBM stack for deque overflow that holds local ports
+#define HVM_LSPILL      1024
 
 // Largest def the evaluator instantiates in one interaction.  Defs are
 // checked at load time.
@@ -60,12 +77,16 @@ struct HvmDevStats {
   // the evaluator runs with HVM_TURN_PROFILE: 0 refill+share, 1 classify+
   // reserve, 2 takes+allocation, 3 creation+release, 4 links, 5 counting
   unsigned long long phase[6];
+  unsigned long long leaks;       // LEAK interactions (local node copied to HBM), not in itrs
+  unsigned long long saves;       // redexes moved out of a local net at the end of a turn
+  unsigned long long local_left;  // local slots still in use after a wave's evacuation
 };
 
 // Error flags (bitwise) reported by the evaluator.
 enum {
   HVM_ERR_OOM_NODE = 1, HVM_ERR_OOM_VARS = 2, HVM_ERR_BAG_OVERFLOW = 4,
   HVM_ERR_LINK_LOOP = 8, HVM_ERR_BAD_DEF = 16, HVM_ERR_BAD_PORT = 32,
+  HVM_ERR_EVAC = 64,    // a local net did not drain
 };
 
 // Per-wave persistent allocator state (cursor into the wave's slices).
@@ -156,6 +177,7 @@ struct HvmTurnArgs {
   HvmSched* sched;
   uint32_t flags;                  // HVM_TURN_* behaviour switches
   uint32_t idle_quit;              // a turn ends once this many waves are idle (0: never early)
+  unsigned long long* lspill;      // local-net evaluator: HVM_LSPILL entries per wave
 };
 
 // Release before the links: the default is the agent-scope release fence;
@@ -163,6 +185,8 @@ struct HvmTurnArgs {
 #define HVM_TURN_WAITCNT_FENCE 1u
 // Accumulate per-phase wall-clock time into HvmDevStats::phase.
 #define HVM_TURN_PROFILE 2u
+// Run the local-net evaluator (node/var allocation in LDS).
+#define HVM_TURN_LOCAL 4u
 
 struct HvmGatherArgs {
   const unsigned long long* node;

@@ -102,8 +102,8 @@ __device__ __forceinline__ u64 wave_sum64(u64 x) {
 
 struct WaveLds {
   u64 deq[HVM_DEQ * HVM_WAVE];  // deque entry j of lane l at [j*64 + l]
-  u32 tbl_n[HVM_TBL_NODE];
-  u32 tbl_v[HVM_TBL_VARS];
+  u32 tbl_n[HVM_TBL_NODE + HVM_TBL_X];
+  u32 tbl_v[HVM_TBL_VARS + HVM_TBL_X];
   u64 inbox[HVM_WAVE];
   u32 target[HVM_WAVE];
   u32 cursor_n, cursor_v;       // allocator scratch
@@ -114,7 +114,93 @@ struct Lane {
   u32 kind[HVM_K_KINDS];
   u64 bytes;
   u32 spills;
+  u32 leaks, saves;             // local-net evaluator only
 };
+
+// ---------------------------------------------------------------------------
+// Local net (LDS) of one wave, for the local-net evaluator.  Slot i of the
+// node (var) buffer is addressed by port value HVM_LOCAL_BASE + i.  A local
+// port is meaningful only inside its wave, so no HBM node or var ever holds
+// one: a link that would store a local node into an HBM var becomes a LEAK
+// interaction, which copies the node to HBM first (src/hvm.cu:1129-1175).
+// Free slots are kept on two LDS stacks.
+// ---------------------------------------------------------------------------
+
+struct LocalNet {
+  u64 node[HVM_LNODE];
+  u32 vars[HVM_LVARS];
+  uint16_t nfree[HVM_LNODE];
+  uint16_t vfree[HVM_LVARS];
+  u32 ntop, vtop;               // free slots on each stack
+  u32 sp;                       // entries on the wave's private HBM spill stack
+};
+static_assert((HVM_LNODE & (HVM_LNODE - 1)) == 0 && HVM_LNODE <= 65536, "HVM_LNODE: power of two <= 2^16");
+static_assert((HVM_LVARS & (HVM_LVARS - 1)) == 0 && HVM_LVARS <= 65536, "HVM_LVARS: power of two <= 2^16");
+static_assert(HVM_GPU_DEF_MAX <= HVM_LNODE && HVM_GPU_DEF_MAX <= HVM_LVARS, "a def must fit a local net");
+
+#define WGS __HIP_MEMORY_SCOPE_WORKGROUP
+
+__device__ __forceinline__ bool is_loc_val(u32 v) { return v >= HVM_LOCAL_BASE && v != HVM_ROOT_VAL; }
+__device__ __forceinline__ bool is_loc_port(u32 p) {
+  u32 t = hvm_get_tag(p);
+  return (t == HVM_VAR || t >= HVM_CON) && is_loc_val(hvm_get_val(p));
+}
+__device__ __forceinline__ u32 loc_n(const HvmTurnArgs& A, u32 v) {
+  u32 i = v - HVM_LOCAL_BASE;
+  if (i >= HVM_LNODE) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT);
+  return i & (HVM_LNODE - 1);
+}
+__device__ __forceinline__ u32 loc_v(const HvmTurnArgs& A, u32 v) {
+  u32 i = v - HVM_LOCAL_BASE;
+  if (i >= HVM_LVARS) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT);
+  return i & (HVM_LVARS - 1);
+}
+
+// Net accessors of the evaluator: local slots in LDS (LOC only), else HBM.
+template <bool LOC>
+__device__ __forceinline__ u64 net_take(const HvmTurnArgs& A, LocalNet* N, u32 loc) {
+  if (LOC && is_loc_val(loc)) return __hip_atomic_exchange(&N->node[loc_n(A, loc)], 0ull, __ATOMIC_RELAXED, WGS);
+  return node_take(A, loc);
+}
+template <bool LOC>
+__device__ __forceinline__ void net_put(const HvmTurnArgs& A, LocalNet* N, u32 loc, u64 v) {
+  if (LOC && is_loc_val(loc)) { __hip_atomic_store(&N->node[loc_n(A, loc)], v, __ATOMIC_RELAXED, WGS); return; }
+  node_put(A, loc, v);
+}
+template <bool LOC>
+__device__ __forceinline__ u32 var_xchg(const HvmTurnArgs& A, LocalNet* N, u32 val, u32 x) {
+  if (LOC && is_loc_val(val)) return __hip_atomic_exchange(&N->vars[loc_v(A, val)], x, __ATOMIC_RELAXED, WGS);
+  return vars_xchg(A, val, x);
+}
+template <bool LOC>
+__device__ __forceinline__ void var_put(const HvmTurnArgs& A, LocalNet* N, u32 val, u32 x) {
+  if (LOC && is_loc_val(val)) { __hip_atomic_store(&N->vars[loc_v(A, val)], x, __ATOMIC_RELAXED, WGS); return; }
+  vars_put(A, val, x);
+}
+// Frees a var slot: HBM slots are zeroed, local slots go back on the stack.
+template <bool LOC>
+__device__ __forceinline__ void var_free(const HvmTurnArgs& A, LocalNet* N, u32 val) {
+  if (LOC && is_loc_val(val)) {
+    u32 i = loc_v(A, val);
+    __hip_atomic_store(&N->vars[i], 0u, __ATOMIC_RELAXED, WGS);
+    u32 p = atomicAdd(&N->vtop, 1u);
+    N->vfree[p & (HVM_LVARS - 1)] = (uint16_t)i;
+    return;
+  }
+  vars_put(A, val, 0u);
+}
+// Frees a node slot whose content was taken or never written.
+template <bool LOC>
+__device__ __forceinline__ void node_free(const HvmTurnArgs& A, LocalNet* N, u32 loc) {
+  if (LOC && is_loc_val(loc)) {
+    u32 i = loc_n(A, loc);
+    __hip_atomic_store(&N->node[i], 0ull, __ATOMIC_RELAXED, WGS);
+    u32 p = atomicAdd(&N->ntop, 1u);
+    N->nfree[p & (HVM_LNODE - 1)] = (uint16_t)i;
+    return;
+  }
+  node_put(A, loc, 0ull);
+}
 
 __device__ __forceinline__ u64& deq_at(WaveLds& L, u32 idx) {
   return L.deq[(idx & (HVM_DEQ - 1)) * HVM_WAVE + lane_id()];
@@ -122,13 +208,22 @@ __device__ __forceinline__ u64& deq_at(WaveLds& L, u32 idx) {
 
 // Pushes a redex on this lane's deque; when full, the oldest redex goes to
 // the HBM output bag.
-__device__ __forceinline__ void push_redex(WaveLds& L, Lane& ln, const HvmTurnArgs& A, u64 r) {
+// A redex holding a local port stays with its wave: with the local net it
+// goes to the wave's private HBM stack instead.
+template <bool LOC>
+__device__ __forceinline__ void push_redex(WaveLds& L, LocalNet* N, Lane& ln, const HvmTurnArgs& A, u64 r) {
   if (ln.top - ln.bot == HVM_DEQ) {
     u64 old = deq_at(L, ln.bot);
     ln.bot++;
-    u32 pos = atomicAdd(&A.ctl->n_out, 1u);
-    if (pos < A.bag_cap) A.bag[A.ctl->parity ^ 1u][pos] = old;
-    else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
+    if (LOC && (is_loc_port(hvm_get_fst(old)) || is_loc_port(hvm_get_snd(old)))) {
+      u32 pos = atomicAdd(&N->sp, 1u);
+      if (pos < HVM_LSPILL) A.lspill[(u64)blockIdx.x * HVM_LSPILL + pos] = old;
+      else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
+    } else {
+      u32 pos = atomicAdd(&A.ctl->n_out, 1u);
+      if (pos < A.bag_cap) A.bag[A.ctl->parity ^ 1u][pos] = old;
+      else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
+    }
     ln.spills++;
   }
   deq_at(L, ln.top) = r;
@@ -141,12 +236,13 @@ __device__ __forceinline__ void push_redex(WaveLds& L, Lane& ln, const HvmTurnAr
 
 #define HVM_LINK_CAP (1u << 16)
 
-__device__ __forceinline__ u32 enter(const HvmTurnArgs& A, u32 var) {
+template <bool LOC>
+__device__ __forceinline__ u32 enter(const HvmTurnArgs& A, LocalNet* N, u32 var) {
   for (u32 it = 0; it < HVM_LINK_CAP; ++it) {
     if (hvm_get_tag(var) != HVM_VAR) return var;
-    u32 val = vars_xchg(A, hvm_get_val(var), HVM_NONE);
+    u32 val = var_xchg<LOC>(A, N, hvm_get_val(var), HVM_NONE);
     if (val == HVM_NONE || val == 0) return var;
-    vars_put(A, hvm_get_val(var), 0);
+    var_free<LOC>(A, N, hvm_get_val(var));
     var = val;
   }
   atomicOr(&A.ctl->err, (u32)HVM_ERR_LINK_LOOP);
@@ -165,7 +261,11 @@ __device__ __forceinline__ u32 enter(const HvmTurnArgs& A, u32 var) {
 #define HVM_LB 4
 enum { LK_DONE = 0, LK_ENTER = 1, LK_PUBLISH = 2, LK_ORIENT = 3 };
 
-__device__ __forceinline__ void link_batch(WaveLds& L, Lane& ln, const HvmTurnArgs& A, u32* pa, u32* pb, u32 n) {
+// With the local net, a pair about to publish an HBM var A with a local B
+// (src/hvm.cu:1061-1072) either swaps (B is a var: the local var points to
+// the HBM one) or ends as a LEAK redex A ~ B (B is a node).
+template <bool LOC>
+__device__ __forceinline__ void link_batch(WaveLds& L, LocalNet* N, Lane& ln, const HvmTurnArgs& A, u32* pa, u32* pb, u32 n) {
   u32 st[HVM_LB];
   u32 got = 0;          // pairs that ended with a redex (pa[i] ~ pb[i])
   bool busy = true;     // some pair is not done
@@ -185,12 +285,26 @@ __device__ __forceinline__ void link_batch(WaveLds& L, Lane& ln, const HvmTurnAr
         }
       }
     }
+    if (LOC) {
+#pragma unroll
+      for (int i = 0; i < HVM_LB; ++i) {
+        if (st[i] == LK_PUBLISH && !is_loc_val(hvm_get_val(pa[i])) && is_loc_port(pb[i])) {
+          if (hvm_get_tag(pb[i]) == HVM_VAR) {
+            u32 x = pa[i]; pa[i] = pb[i]; pb[i] = x;
+            st[i] = LK_ENTER;                       // B is now the HBM var
+          } else {
+            st[i] = LK_DONE;                        // LEAK redex
+            got |= 1u << i;
+          }
+        }
+      }
+    }
     u32 r[HVM_LB];
 #pragma unroll
     for (int i = 0; i < HVM_LB; ++i) {
       r[i] = 0;
-      if (st[i] == LK_ENTER) r[i] = vars_xchg(A, hvm_get_val(pb[i]), HVM_NONE);   // extend B
-      else if (st[i] == LK_PUBLISH) r[i] = vars_xchg(A, hvm_get_val(pa[i]), pb[i]);  // A ~> B
+      if (st[i] == LK_ENTER) r[i] = var_xchg<LOC>(A, N, hvm_get_val(pb[i]), HVM_NONE);   // extend B
+      else if (st[i] == LK_PUBLISH) r[i] = var_xchg<LOC>(A, N, hvm_get_val(pa[i]), pb[i]);  // A ~> B
     }
     busy = false;
 #pragma unroll
@@ -199,7 +313,7 @@ __device__ __forceinline__ void link_batch(WaveLds& L, Lane& ln, const HvmTurnAr
         if (r[i] == HVM_NONE || r[i] == 0) {
           st[i] = LK_PUBLISH;                       // B is an unbound var
         } else {
-          vars_put(A, hvm_get_val(pb[i]), 0);       // we own both: free B
+          var_free<LOC>(A, N, hvm_get_val(pb[i]));  // we own both: free B
           pb[i] = r[i];
           st[i] = hvm_get_tag(pb[i]) == HVM_VAR ? (u32)LK_ENTER : (u32)LK_PUBLISH;
         }
@@ -207,7 +321,7 @@ __device__ __forceinline__ void link_batch(WaveLds& L, Lane& ln, const HvmTurnAr
         if (r[i] == HVM_NONE) {
           st[i] = LK_DONE;                          // the other side will find B
         } else {
-          vars_put(A, hvm_get_val(pa[i]), 0);       // A was bound: free it, link A' ~ B
+          var_free<LOC>(A, N, hvm_get_val(pa[i]));  // A was bound: free it, link A' ~ B
           pa[i] = r[i];
           st[i] = LK_ORIENT;
         }
@@ -219,7 +333,7 @@ __device__ __forceinline__ void link_batch(WaveLds& L, Lane& ln, const HvmTurnAr
   if (__builtin_expect(busy, 0)) atomicOr(&A.ctl->err, (u32)HVM_ERR_LINK_LOOP);
 #pragma unroll
   for (int i = 0; i < HVM_LB; ++i)
-    if (got & (1u << i)) push_redex(L, ln, A, hvm_new_pair(pa[i], pb[i]));
+    if (got & (1u << i)) push_redex<LOC>(L, N, ln, A, hvm_new_pair(pa[i], pb[i]));
 }
 
 // ---------------------------------------------------------------------------
@@ -289,7 +403,8 @@ __device__ __forceinline__ bool scan_advance(Scan& s, u32 pool_lo, u32 pool_len,
 // round issues one coalesced probe load per buffer that still needs slots, so
 // both buffers are scanned concurrently.  All arguments are wave-uniform.
 __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u32 n_lo, u32 v_lo,
-                                            HvmWaveState& cur, u32 need_n, u32 need_v, u64& probes, u32& oom) {
+                                            HvmWaveState& cur, u32 need_n, u32 need_v, u64& probes, u32& oom,
+                                            u32 tbl_base) {
   const u32 lane = lane_id();
   Scan sn = {n_lo, A.node_slice, cur.node_cur, 0, need_n, 0, false};
   Scan sv = {v_lo, A.vars_slice, cur.vars_cur, 0, need_v, 0, false};
@@ -303,12 +418,12 @@ __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u3
     u64 vn = pn ? __hip_atomic_load(&A.node[in_], __ATOMIC_RELAXED, AGENT) : 1ull;
     u32 vv = pv ? __hip_atomic_load(&A.vars[iv], __ATOMIC_RELAXED, AGENT) : 1u;
     if (pn) {
-      scan_window(A.node, sn, vn, in_, rn, 1u, ~0ull, L.tbl_n, &L.cursor_n);
+      scan_window(A.node, sn, vn, in_, rn, 1u, ~0ull, L.tbl_n + tbl_base, &L.cursor_n);
       probes++;
       if (!scan_advance(sn, A.node_pool_lo, A.node_cap - A.node_pool_lo, cur.node_ovf)) { oom = HVM_ERR_OOM_NODE; return false; }
     }
     if (pv) {
-      scan_window(A.vars, sv, vv, iv, rv, A.root_vars, (u32)HVM_NONE, L.tbl_v, &L.cursor_v);
+      scan_window(A.vars, sv, vv, iv, rv, A.root_vars, (u32)HVM_NONE, L.tbl_v + tbl_base, &L.cursor_v);
       probes++;
       if (!scan_advance(sv, A.vars_pool_lo, A.vars_cap - A.vars_pool_lo, cur.vars_ovf)) { oom = HVM_ERR_OOM_VARS; return false; }
     }
@@ -324,8 +439,18 @@ __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u3
 // The turn kernel
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTurnArgs A) {
-  __shared__ WaveLds L;
+// Extra rules of the local-net evaluator (device-only codes).
+enum {
+  HVM_LEAK = 8,   // HBM var ~ local node: copy the node to HBM (src/hvm.cu:1129-1175)
+  HVM_SAVE = 9,   // end of turn: a redex with local ports gets HBM vars (src/hvm.cu:1426-1446)
+  HVM_EMIT = 10,  // end of turn: a redex of HBM ports goes to the output bag as is
+};
+
+// Steps a wave may spend draining its local net before giving up.
+#define HVM_EVAC_CAP (1u << 16)
+
+template <bool LOC>
+__device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, LocalNet* N) {
   const u32 lane = lane_id();
   const u32 w = blockIdx.x;
 
@@ -344,7 +469,16 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
   for (int k = 0; k < HVM_K_KINDS; ++k) ln.kind[k] = 0;
   ln.bytes = 0;
   ln.spills = 0;
+  ln.leaks = ln.saves = 0;
   u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0;
+
+  if (LOC) {
+    // an empty local net: every slot free
+    for (u32 i = lane; i < HVM_LNODE; i += HVM_WAVE) { N->node[i] = 0; N->nfree[i] = (uint16_t)(HVM_LNODE - 1 - i); }
+    for (u32 i = lane; i < HVM_LVARS; i += HVM_WAVE) { N->vars[i] = 0; N->vfree[i] = (uint16_t)(HVM_LVARS - 1 - i); }
+    if (lane == 0) { N->ntop = HVM_LNODE; N->vtop = HVM_LVARS; N->sp = 0; }
+    __syncthreads();
+  }
 
   // Slices start after the reserved slots: node slot 0, var slots [0, root_vars).
   const u32 n_lo = 1 + w * A.node_slice, v_lo = A.root_vars + w * A.vars_slice;
@@ -361,9 +495,13 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
   // spread one redex per wave before any wave takes a second one.
   {
     u32 idx = w + lane * A.nwaves;
-    if (idx < n_in && idx < HVM_WAVE * A.nwaves) push_redex(L, ln, A, bag_in[idx]);
+    if (idx < n_in && idx < HVM_WAVE * A.nwaves) push_redex<LOC>(L, N, ln, A, bag_in[idx]);
   }
   bool more = n_in > HVM_WAVE * A.nwaves;
+  // With the local net a wave ends its turn by draining the net (evacuation):
+  // high-priority redexes run, every other redex leaves through the output bag.
+  bool evac = false;
+  u32 evac_steps = 0;
 
   // optional per-phase timing (HVM_TURN_PROFILE), wall clock at 100 MHz
   const bool prof = (A.flags & HVM_TURN_PROFILE) != 0;
@@ -376,10 +514,33 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
     t_prev = t_now;                         \
   }
 
-  for (u32 step = 0; step < max_steps; ++step) {
+  for (u32 step = 0;; ++step) {
+    if (!evac && step >= max_steps) {
+      if (!LOC) break;
+      evac = true;
+    }
+    if (evac && ++evac_steps > HVM_EVAC_CAP) {
+      if (lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_EVAC);
+      break;
+    }
     if (prof) t_prev = wall_clock64();
-    // ---- refill from the input bag ----
-    if (more) {
+    // ---- refill: the wave's private spill stack first, then the input bag ----
+    if (LOC) {
+      u32 sp = min(__hip_atomic_load(&N->sp, __ATOMIC_RELAXED, WGS), (u32)HVM_LSPILL);
+      if (sp) {
+        bool empty = ln.top == ln.bot;
+        u64 em = __ballot(empty);
+        u32 k = min((u32)__popcll(em), sp);
+        u32 r = rank_in(em);
+        u64 got = 0;
+        if (empty && r < k) got = A.lspill[(u64)w * HVM_LSPILL + (sp - 1 - r)];
+        __syncthreads();
+        if (lane == 0) __hip_atomic_store(&N->sp, sp - k, __ATOMIC_RELAXED, WGS);
+        __syncthreads();
+        if (empty && r < k) push_redex<LOC>(L, N, ln, A, got);
+      }
+    }
+    if (more && !evac) {
       bool empty = ln.top == ln.bot;
       u64 em = __ballot(empty);
       if (em) {
@@ -389,7 +550,7 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
         base = __shfl(base, 0, HVM_WAVE);
         if (base >= n_in) more = false;
         u32 r = rank_in(em);
-        if (empty && base + r < n_in) push_redex(L, ln, A, bag_in[base + r]);
+        if (empty && base + r < n_in) push_redex<LOC>(L, N, ln, A, bag_in[base + r]);
       }
     }
 
@@ -409,23 +570,26 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
           ln.bot++;
         }
         __syncthreads();
-        if (needy && rn < k) push_redex(L, ln, A, L.inbox[lane]);
+        if (needy && rn < k) push_redex<LOC>(L, N, ln, A, L.inbox[lane]);
       }
     }
 
     bool has = ln.top != ln.bot;
     if (!__ballot(has)) {
-      if (!more) {
-        // out of work: count this wave idle, so that busy waves end the turn
-        // early and their redexes get redistributed
-        if (lane == 0) atomicAdd(&A.ctl->idle_waves, 1u);
-        break;
-      }
-      continue;
+      if (more && !evac) continue;
+      // out of work: count this wave idle, so that busy waves end the turn
+      // early and their redexes get redistributed
+      if (lane == 0) atomicAdd(&A.ctl->idle_waves, 1u);
+      break;
     }
     if ((step & 15) == 15) {
-      u32 idle = __hip_atomic_load(&A.ctl->idle_waves, __ATOMIC_RELAXED, AGENT);
-      if (A.idle_quit && idle >= A.idle_quit) break;
+      if (!evac) {
+        u32 idle = __hip_atomic_load(&A.ctl->idle_waves, __ATOMIC_RELAXED, AGENT);
+        if (A.idle_quit && idle >= A.idle_quit) {
+          if (!LOC) break;
+          evac = true;
+        }
+      }
       if ((step & 63) == 63 && __hip_atomic_load(&A.ctl->err, __ATOMIC_RELAXED, AGENT) != 0) break;
     }
     steps++;
@@ -441,10 +605,19 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
     if (hvm_get_tag(a) == HVM_REF && hvm_get_tag(b) == HVM_VAR &&
         (b == HVM_ROOT || (hvm_get_val(b) != 0 && hvm_get_val(b) < A.root_vars))) rule = HVM_CALL;
     else if (hvm_should_swap(a, b)) { u32 x = a; a = b; b = x; }
+    if (LOC) {
+      // an HBM var against a local node
+      if (rule == HVM_LINK && hvm_get_tag(a) == HVM_VAR && !is_loc_val(hvm_get_val(a)) &&
+          hvm_get_tag(b) >= HVM_CON && is_loc_val(hvm_get_val(b))) rule = HVM_LEAK;
+      // draining: low-priority redexes leave the wave
+      if (evac && (rule == HVM_CALL || rule == HVM_COMM || rule == HVM_OPER || rule == HVM_SWIT))
+        rule = (is_loc_port(a) || is_loc_port(b)) ? HVM_SAVE : HVM_EMIT;
+    }
     bool copy = false;
     HvmDevDef d;
     d.rlen = d.nlen = d.vlen = 0; d.rbag_off = d.node_off = 0; d.root = 0; d.safe = 0; d.call_bytes = 0;
-    u32 nn = 0, nv = 0;
+    u32 nn = 0, nv = 0;   // slots from the net's main allocator (local with LOC)
+    u32 xn = 0, xv = 0;   // HBM slots of LEAK / SAVE
     if (has) {
       if (rule == HVM_CALL) {
         u32 fid = hvm_get_val(a) & 0xFFFFFFFu;
@@ -454,16 +627,39 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
       } else if (rule == HVM_COMM) { nn = 4; nv = 4; }
       else if (rule == HVM_OPER) { nn = 1; }
       else if (rule == HVM_SWIT) { nn = hvm_get_u24(hvm_get_val(a)) == 0 ? 1 : 2; }
+      else if (LOC && rule == HVM_LEAK) { xn = 1; xv = 2; }
+      else if (LOC && rule == HVM_SAVE) { xv = (u32)is_loc_port(a) + (u32)is_loc_port(b); }
     }
 
     // ---- P1: reserve allocation-table space (lanes that do not fit wait) ----
-    u32 tot_n, tot_v;
-    u32 off_n = wave_scan(nn, &tot_n);
-    u32 off_v = wave_scan(nv, &tot_v);
-    bool run = has && off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
+    // node and var counts are scanned together (16 bits each; a wave's total
+    // is at most 64 * HVM_GPU_DEF_MAX < 2^16)
+    u32 tot_m, tot_x = 0;
+    u32 off_m = wave_scan(nn | (nv << 16), &tot_m);
+    u32 off_n = off_m & 0xFFFFu, off_v = off_m >> 16;
+    u32 off_xn = 0, off_xv = 0;
+    u32 cap_n = HVM_TBL_NODE, cap_v = HVM_TBL_VARS;
+    if (LOC) {
+      u32 off_x = wave_scan(xn | (xv << 16), &tot_x);
+      off_xn = off_x & 0xFFFFu; off_xv = off_x >> 16;
+      cap_n = min(cap_n, __hip_atomic_load(&N->ntop, __ATOMIC_RELAXED, WGS));
+      cap_v = min(cap_v, __hip_atomic_load(&N->vtop, __ATOMIC_RELAXED, WGS));
+    }
+    bool run = has && off_n + nn <= cap_n && off_v + nv <= cap_v &&
+               off_xn + xn <= HVM_TBL_X && off_xv + xv <= HVM_TBL_X;
     if (has && !run) defers++;
+    if (LOC && !evac && !__ballot(run)) {
+      // the local net is too full for any redex here: drain it
+      evac = true;
+      continue;
+    }
     u32 need_n = wave_max(run ? off_n + nn : 0);
     u32 need_v = wave_max(run ? off_v + nv : 0);
+    u32 need_xn = 0, need_xv = 0;
+    if (LOC && tot_x) {
+      need_xn = wave_max(run ? off_xn + xn : 0);
+      need_xv = wave_max(run ? off_xv + xv : 0);
+    }
     if (run) ln.top--;
 
     HVM_PHASE(1);
@@ -471,53 +667,87 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
     //      round trips overlap ----
     bool take_a = run && (rule == HVM_ANNI || rule == HVM_COMM);
     bool take_b = run && (rule == HVM_ERAS || rule == HVM_ANNI || rule == HVM_COMM ||
-                          rule == HVM_OPER || rule == HVM_SWIT || (rule == HVM_CALL && copy));
-    u64 NA = take_a ? node_take(A, hvm_get_val(a)) : 0ull;
-    u64 NB = take_b ? node_take(A, hvm_get_val(b)) : 0ull;
+                          rule == HVM_OPER || rule == HVM_SWIT || (rule == HVM_CALL && copy) ||
+                          (LOC && rule == HVM_LEAK));
+    u64 NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
+    u64 NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
 
     // ---- P2: allocation (wave-cooperative, nodes and vars together) ----
-    if (need_n || need_v) {
+    if (LOC) {
+      // local slots come off the free stacks, top first
+      if (need_n) {
+        u32 t = __hip_atomic_load(&N->ntop, __ATOMIC_RELAXED, WGS);
+        for (u32 i = lane; i < need_n; i += HVM_WAVE) L.tbl_n[i] = HVM_LOCAL_BASE + N->nfree[(t - 1 - i) & (HVM_LNODE - 1)];
+        __syncthreads();
+        if (lane == 0) __hip_atomic_store(&N->ntop, t - need_n, __ATOMIC_RELAXED, WGS);
+      }
+      if (need_v) {
+        u32 t = __hip_atomic_load(&N->vtop, __ATOMIC_RELAXED, WGS);
+        for (u32 i = lane; i < need_v; i += HVM_WAVE) L.tbl_v[i] = HVM_LOCAL_BASE + N->vfree[(t - 1 - i) & (HVM_LVARS - 1)];
+        __syncthreads();
+        if (lane == 0) __hip_atomic_store(&N->vtop, t - need_v, __ATOMIC_RELAXED, WGS);
+      }
+      __syncthreads();
+      if (need_xn || need_xv) {
+        u32 oom = 0;
+        if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_xn, need_xv, probes, oom, HVM_TBL_NODE)) {
+          atomicOr(&A.ctl->err, oom);
+          if (run) ln.top++;
+          break;
+        }
+      }
+    } else if (need_n || need_v) {
       u32 oom = 0;
-      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom)) {
+      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom, 0)) {
         // out of slots in the slice and the pool: the evaluation stops with an error
         atomicOr(&A.ctl->err, oom);
         if (run) ln.top++;
         break;
       }
     }
+    const u32* tn = L.tbl_n + off_n;
+    const u32* tv = L.tbl_v + off_v;
+    const u32* gn = L.tbl_n + HVM_TBL_NODE + off_xn;   // HBM slots of LEAK / SAVE
+    const u32* gv = L.tbl_v + HVM_TBL_VARS + off_xv;
 
     if ((take_a && NA == 0) || (take_b && NB == 0)) {
       // A node this redex points to is not visible yet: put everything back
       // and retry the redex on a later step (src/hvm.c:1112-1114).
-      if (take_a && NA) node_put(A, hvm_get_val(a), NA);
-      if (take_b && NB) node_put(A, hvm_get_val(b), NB);
+      if (take_a && NA) net_put<LOC>(A, N, hvm_get_val(a), NA);
+      if (take_b && NB) net_put<LOC>(A, N, hvm_get_val(b), NB);
       // slots allocated for this interaction go back (they may be pool claims)
-      for (u32 i = 0; i < nn; ++i) node_put(A, L.tbl_n[off_n + i], 0ull);
-      for (u32 i = 0; i < nv; ++i) vars_put(A, L.tbl_v[off_v + i], 0u);
+      for (u32 i = 0; i < nn; ++i) node_free<LOC>(A, N, tn[i]);
+      for (u32 i = 0; i < nv; ++i) var_free<LOC>(A, N, tv[i]);
+      for (u32 i = 0; i < xn; ++i) node_put(A, gn[i], 0ull);
+      for (u32 i = 0; i < xv; ++i) vars_put(A, gv[i], 0u);
       ln.top++;
       run = false;
       retries++;
+    }
+    // taken local nodes are dead now: their slots go back on the stack
+    if (LOC && run) {
+      if (take_a && is_loc_val(hvm_get_val(a))) node_free<LOC>(A, N, hvm_get_val(a));
+      if (take_b && is_loc_val(hvm_get_val(b))) node_free<LOC>(A, N, hvm_get_val(b));
     }
 
     u32 B1 = hvm_get_fst(NB), B2 = hvm_get_snd(NB);
     bool oper1 = false;
     u32 cv = 0;
     if (run && rule == HVM_OPER) {
-      B2 = enter(A, B2);
+      B2 = enter<LOC>(A, N, B2);
       oper1 = hvm_get_tag(B1) == HVM_NUM;
       if (oper1) {
         cv = hvm_operate(hvm_get_val(a), hvm_get_val(B1));
-        node_put(A, L.tbl_n[off_n], 0ull);  // the node reserved for OPER-2 is not needed
+        node_free<LOC>(A, N, tn[0]);  // the node reserved for OPER-2 is not needed
       }
     }
 
     HVM_PHASE(2);
     // ---- P4: creation ----
-    const u32* tn = L.tbl_n + off_n;
-    const u32* tv = L.tbl_v + off_v;
+    u32 g1 = 0, g2 = 0;   // LEAK: HBM stand-ins for the node's ports
     if (run) {
       if (rule == HVM_CALL && !copy) {
-        for (u32 i = 0; i < d.vlen; ++i) vars_put(A, tv[i], HVM_NONE);
+        for (u32 i = 0; i < d.vlen; ++i) var_put<LOC>(A, N, tv[i], HVM_NONE);
         for (u32 i = 0; i < d.nlen; ++i) {
           u64 p = A.pairs[d.node_off + i];
           u32 p1 = hvm_get_fst(p), p2 = hvm_get_snd(p);
@@ -525,27 +755,39 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
           else if (hvm_is_var(p1)) p1 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p1)]);
           if (hvm_is_nod(p2)) p2 = hvm_new_port(hvm_get_tag(p2), tn[hvm_get_val(p2)]);
           else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
-          node_put(A, tn[i], hvm_new_pair(p1, p2));
+          net_put<LOC>(A, N, tn[i], hvm_new_pair(p1, p2));
         }
       } else if (rule == HVM_COMM) {
-        vars_put(A, tv[0], HVM_NONE);
-        vars_put(A, tv[1], HVM_NONE);
-        vars_put(A, tv[2], HVM_NONE);
-        vars_put(A, tv[3], HVM_NONE);
-        node_put(A, tn[0], hvm_new_pair(hvm_new_port(HVM_VAR, tv[0]), hvm_new_port(HVM_VAR, tv[1])));
-        node_put(A, tn[1], hvm_new_pair(hvm_new_port(HVM_VAR, tv[2]), hvm_new_port(HVM_VAR, tv[3])));
-        node_put(A, tn[2], hvm_new_pair(hvm_new_port(HVM_VAR, tv[0]), hvm_new_port(HVM_VAR, tv[2])));
-        node_put(A, tn[3], hvm_new_pair(hvm_new_port(HVM_VAR, tv[1]), hvm_new_port(HVM_VAR, tv[3])));
+        var_put<LOC>(A, N, tv[0], HVM_NONE);
+        var_put<LOC>(A, N, tv[1], HVM_NONE);
+        var_put<LOC>(A, N, tv[2], HVM_NONE);
+        var_put<LOC>(A, N, tv[3], HVM_NONE);
+        net_put<LOC>(A, N, tn[0], hvm_new_pair(hvm_new_port(HVM_VAR, tv[0]), hvm_new_port(HVM_VAR, tv[1])));
+        net_put<LOC>(A, N, tn[1], hvm_new_pair(hvm_new_port(HVM_VAR, tv[2]), hvm_new_port(HVM_VAR, tv[3])));
+        net_put<LOC>(A, N, tn[2], hvm_new_pair(hvm_new_port(HVM_VAR, tv[0]), hvm_new_port(HVM_VAR, tv[2])));
+        net_put<LOC>(A, N, tn[3], hvm_new_pair(hvm_new_port(HVM_VAR, tv[1]), hvm_new_port(HVM_VAR, tv[3])));
       } else if (rule == HVM_OPER && !oper1) {
-        node_put(A, tn[0], hvm_new_pair(a, B2));
+        net_put<LOC>(A, N, tn[0], hvm_new_pair(a, B2));
       } else if (rule == HVM_SWIT) {
         u32 av = hvm_get_u24(hvm_get_val(a));
         if (av == 0) {
-          node_put(A, tn[0], hvm_new_pair(B2, hvm_new_port(HVM_ERA, 0)));
+          net_put<LOC>(A, N, tn[0], hvm_new_pair(B2, hvm_new_port(HVM_ERA, 0)));
         } else {
-          node_put(A, tn[0], hvm_new_pair(hvm_new_port(HVM_ERA, 0), hvm_new_port(HVM_CON, tn[1])));
-          node_put(A, tn[1], hvm_new_pair(hvm_new_port(HVM_NUM, hvm_new_u24(av - 1)), B2));
+          net_put<LOC>(A, N, tn[0], hvm_new_pair(hvm_new_port(HVM_ERA, 0), hvm_new_port(HVM_CON, tn[1])));
+          net_put<LOC>(A, N, tn[1], hvm_new_pair(hvm_new_port(HVM_NUM, hvm_new_u24(av - 1)), B2));
         }
+      } else if (LOC && rule == HVM_LEAK) {
+        // The node's ports, with local ones replaced by fresh HBM vars.  The
+        // ports are copied as they are, not entered: whether a port is a var
+        // decides later rules (OPER-1 needs a NUM in the fst port itself,
+        // src/hvm.c:1023-1028), and ITRS must stay that of run-c.
+        if (is_loc_port(B1)) { g1 = hvm_new_port(HVM_VAR, gv[0]); vars_put(A, gv[0], HVM_NONE); }
+        else { g1 = B1; vars_put(A, gv[0], 0u); }
+        if (is_loc_port(B2)) { g2 = hvm_new_port(HVM_VAR, gv[1]); vars_put(A, gv[1], HVM_NONE); }
+        else { g2 = B2; vars_put(A, gv[1], 0u); }
+        node_put(A, gn[0], hvm_new_pair(g1, g2));
+      } else if (LOC && rule == HVM_SAVE) {
+        for (u32 i = 0; i < xv; ++i) vars_put(A, gv[i], HVM_NONE);
       }
     }
 
@@ -560,6 +802,7 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
 
     HVM_PHASE(3);
     // ---- P6: links ----
+    u64 emit = 0;         // SAVE / EMIT: the redex that leaves through the output bag
     if (run) {
       u32 pa[HVM_LB], pb[HVM_LB], np = 0;
       switch (rule) {
@@ -587,7 +830,7 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
                 else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
               }
               pa[np] = p1; pb[np] = p2; np++;
-              if (np == HVM_LB && i < d.rlen) { link_batch(L, ln, A, pa, pb, np); np = 0; }
+              if (np == HVM_LB && i < d.rlen) { link_batch<LOC>(L, N, ln, A, pa, pb, np); np = 0; }
             }
           }
           break;
@@ -613,9 +856,21 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
           pa[0] = hvm_new_port(HVM_CON, tn[0]); pb[0] = B1; np = 1;
           break;
         default:
+          if (LOC && rule == HVM_LEAK) {
+            if (is_loc_port(B1)) { pa[np] = g1; pb[np] = B1; np++; }
+            if (is_loc_port(B2)) { pa[np] = g2; pb[np] = B2; np++; }
+            pa[np] = a; pb[np] = hvm_new_port(hvm_get_tag(b), gn[0]); np++;
+          } else if (LOC && rule == HVM_SAVE) {
+            u32 k = 0, x = a, y = b;
+            if (is_loc_port(a)) { x = hvm_new_port(HVM_VAR, gv[k]); pa[np] = x; pb[np] = a; np++; k++; }
+            if (is_loc_port(b)) { y = hvm_new_port(HVM_VAR, gv[k]); pa[np] = y; pb[np] = b; np++; k++; }
+            emit = hvm_new_pair(x, y);
+          } else if (LOC && rule == HVM_EMIT) {
+            emit = hvm_new_pair(a, b);
+          }
           break;
       }
-      if (np) link_batch(L, ln, A, pa, pb, np);
+      if (np) link_batch<LOC>(L, N, ln, A, pa, pb, np);
 
       // ---- P7: counting ----
       switch (rule) {
@@ -635,7 +890,25 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
           if (hvm_get_u24(hvm_get_val(a)) == 0) { ln.kind[HVM_K_SWIT0]++; ln.bytes += 32; }
           else { ln.kind[HVM_K_SWITN]++; ln.bytes += 40; }
           break;
+        case HVM_LEAK: ln.leaks++; break;
+        case HVM_SAVE: ln.saves++; break;
         default: break;
+      }
+    }
+    if (LOC && evac) {
+      // redexes leaving the wave go to the output bag, one atomic per wave
+      bool out1 = emit != 0;
+      u64 om = __ballot(out1);
+      if (om) {
+        u32 k = __popcll(om);
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(&A.ctl->n_out, k);
+        base = __shfl(base, 0, HVM_WAVE);
+        u32 pos = base + rank_in(om);
+        if (out1) {
+          if (pos < A.bag_cap) bag_out[pos] = emit;
+          else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
+        }
       }
     }
     HVM_PHASE(4);
@@ -655,6 +928,15 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
     } else {
       for (u32 i = 0; i < n; ++i) bag_out[base + off + i] = deq_at(L, ln.bot + i);
     }
+  }
+  u64 left = 0;
+  if (LOC) {
+    // a drained wave holds no redex; what the local net still holds is garbage
+    // (a cycle no redex reaches) or, after an error, the net's lost part
+    u32 sp = __hip_atomic_load(&N->sp, __ATOMIC_RELAXED, WGS);
+    left = (u64)(HVM_LNODE - __hip_atomic_load(&N->ntop, __ATOMIC_RELAXED, WGS)) +
+           (u64)(HVM_LVARS - __hip_atomic_load(&N->vtop, __ATOMIC_RELAXED, WGS));
+    if ((ln.top != ln.bot || sp != 0) && lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_EVAC);
   }
 
   // ---- input redexes no wave pulled this turn are carried to the output ----
@@ -684,6 +966,8 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
     u64 s_spill = wave_sum64(ln.spills);
     defers = wave_sum64(defers);    // per-lane counts; steps/probes/lane_steps are wave-uniform
     retries = wave_sum64(retries);
+    u64 s_leaks = 0, s_saves = 0;
+    if (LOC) { s_leaks = wave_sum64(ln.leaks); s_saves = wave_sum64(ln.saves); }
     u64 kinds[HVM_K_KINDS];
 #pragma unroll
     for (int k = 0; k < HVM_K_KINDS; ++k) kinds[k] = wave_sum64(ln.kind[k]);
@@ -697,6 +981,9 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
       if (defers) atomicAdd(&A.stats->defers, defers);
       if (retries) atomicAdd(&A.stats->retries, retries);
       if (probes) atomicAdd(&A.stats->probes, probes);
+      if (s_leaks) atomicAdd(&A.stats->leaks, s_leaks);
+      if (s_saves) atomicAdd(&A.stats->saves, s_saves);
+      if (left) atomicAdd(&A.stats->local_left, left);
       if (prof) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) if (ph[k]) atomicAdd(&A.stats->phase[k], ph[k]);
@@ -714,6 +1001,18 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
       for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&T->kind[k], kinds[k]);
     }
   }
+}
+
+__global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTurnArgs A) {
+  __shared__ WaveLds L;
+  turn_body<false>(A, L, nullptr);
+}
+
+// The local-net evaluator: the same turn with node/var allocation in LDS.
+__global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_local_kernel(HvmTurnArgs A) {
+  __shared__ WaveLds L;
+  __shared__ LocalNet N;
+  turn_body<true>(A, L, &N);
 }
 
 // ---------------------------------------------------------------------------
@@ -904,7 +1203,10 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
 // ---------------------------------------------------------------------------
 
 extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s) {
-  hipLaunchKernelGGL(hvm_turn_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
+  if (a->flags & HVM_TURN_LOCAL)
+    hipLaunchKernelGGL(hvm_turn_local_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
+  else
+    hipLaunchKernelGGL(hvm_turn_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
   return hipGetLastError();
 }
 
@@ -927,12 +1229,13 @@ extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s) {
   return hipGetLastError();
 }
 
-extern "C" size_t hvm_turn_lds_bytes(void) { return sizeof(WaveLds); }
+extern "C" size_t hvm_turn_lds_bytes(int local) { return sizeof(WaveLds) + (local ? sizeof(LocalNet) : 0); }
 
 // Turn-kernel workgroups (one wave each) that can be resident on one CU at
 // once, given the kernel's registers and LDS; 0 if the runtime cannot tell.
-extern "C" int hvm_turn_waves_per_cu(void) {
+extern "C" int hvm_turn_waves_per_cu(int local) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, hvm_turn_kernel, HVM_WAVE, 0) != hipSuccess) return 0;
-  return n;
+  hipError_t e = local ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, hvm_turn_local_kernel, HVM_WAVE, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, hvm_turn_kernel, HVM_WAVE, 0);
+  return e == hipSuccess ? n : 0;
 }

@@ -20,8 +20,8 @@ extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s);
 extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, uint32_t clock_khz, hipStream_t s);
 extern "C" uint32_t hvm_plain_action_host(uint32_t n, uint32_t nwaves);
 extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves);
-extern "C" size_t hvm_turn_lds_bytes(void);
-extern "C" int hvm_turn_waves_per_cu(void);
+extern "C" size_t hvm_turn_lds_bytes(int local);
+extern "C" int hvm_turn_waves_per_cu(int local);
 
 namespace {
 
@@ -58,6 +58,7 @@ struct hvm_hip_ctx {
   u32 bag_cap = 0;
   HvmCtl* ctl = nullptr;       // turn control block
   HvmWaveState* wave_state = nullptr;
+  u64* lspill = nullptr;
   HvmDevStats* stats = nullptr;
   // scheduler
   HvmSched* sched = nullptr;
@@ -93,6 +94,9 @@ struct hvm_hip_ctx {
     uint64_t ncap = cfg.node_cap ? cfg.node_cap : (1ull << 28);
     uint64_t vcap = cfg.vars_cap ? cfg.vars_cap : (1ull << 28);
     if (ncap > (1ull << 29) || vcap > (1ull << 29)) return fail("node_cap/vars_cap exceed 2^29");
+    const bool local = (cfg.flags & HVM_HIP_LOCAL_NET) != 0;
+    if (local && (ncap > HVM_LOCAL_BASE || vcap > HVM_LOCAL_BASE))
+      return fail("node_cap/vars_cap exceed 2^28, the limit of the local-net evaluator");
     node_cap = (u32)ncap;
     vars_cap = (u32)vcap;
     bag_cap = cfg.bag_cap ? cfg.bag_cap : (1u << 24);
@@ -101,8 +105,8 @@ struct hvm_hip_ctx {
     } else {
       // Fill the chip: as many one-wave workgroups as can be resident per CU
       // (registers and LDS), so that every wave of a turn runs at once.
-      int occ = hvm_turn_waves_per_cu();
-      size_t lds = hvm_turn_lds_bytes();
+      int occ = hvm_turn_waves_per_cu(local ? 1 : 0);
+      size_t lds = hvm_turn_lds_bytes(local ? 1 : 0);
       size_t per_cu = occ > 0 ? (size_t)occ : (160u * 1024u) / (lds ? lds : 1);
       if (per_cu < 1) per_cu = 1;
       if (per_cu > 16) per_cu = 16;
@@ -127,6 +131,7 @@ struct hvm_hip_ctx {
     if (!ck(hipMalloc(&bag[1], (size_t)bag_cap * 8), "hipMalloc(bag)")) return false;
     if (!ck(hipMalloc(&ctl, sizeof(HvmCtl)), "hipMalloc(ctl)")) return false;
     if (!ck(hipMalloc(&wave_state, nwaves * sizeof(HvmWaveState)), "hipMalloc(wave_state)")) return false;
+    if (local && !ck(hipMalloc(&lspill, (size_t)nwaves * HVM_LSPILL * 8), "hipMalloc(lspill)")) return false;
     if (!ck(hipMalloc(&stats, sizeof(HvmDevStats)), "hipMalloc(stats)")) return false;
     if (!ck(hipMalloc(&sched, sizeof(HvmSched)), "hipMalloc(sched)")) return false;
     if (!ck(hipMalloc(&d_params, HVM_MLP_PARAMS * sizeof(unsigned short)), "hipMalloc(params)")) return false;
@@ -144,6 +149,7 @@ struct hvm_hip_ctx {
     if (bag[1]) hipFree(bag[1]);
     if (ctl) hipFree(ctl);
     if (wave_state) hipFree(wave_state);
+    if (lspill) hipFree(lspill);
     if (stats) hipFree(stats);
     if (sched) hipFree(sched);
     if (d_params) hipFree(d_params);
@@ -334,6 +340,8 @@ struct hvm_hip_ctx {
     a.sched = sched;
     a.flags = (cfg.flags & HVM_HIP_FENCE_RELEASE) ? 0u : HVM_TURN_WAITCNT_FENCE;
     if (cfg.flags & HVM_HIP_PROFILE) a.flags |= HVM_TURN_PROFILE;
+    if (cfg.flags & HVM_HIP_LOCAL_NET) a.flags |= HVM_TURN_LOCAL;
+    a.lspill = lspill;
     // a turn ends early once idle_pct% of the waves ran out of work (default 50%)
     u32 pct = cfg.idle_pct ? (cfg.idle_pct > 100 ? 0u : cfg.idle_pct) : 50u;
     a.idle_quit = pct ? std::max<u32>(1u, (u32)(((u64)nwaves * pct) / 100)) : 0u;
@@ -560,6 +568,7 @@ struct hvm_hip_ctx {
       out->launches = launches;
       out->launch_ms = launch_ms;
       out->waves = nwaves;
+      out->leak = s.leaks; out->saves = s.saves; out->local_left = s.local_left;
       out->normalizes = normalizes;
     }
     if (rc != 0) {
@@ -623,6 +632,7 @@ struct hvm_hip_ctx {
     out->launches = launches;
     out->launch_ms = launch_ms;
     out->waves = nwaves;
+    out->leak = s.leaks; out->saves = s.saves; out->local_left = s.local_left;
     out->normalizes = normalizes;
   }
 

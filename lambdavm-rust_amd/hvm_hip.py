@@ -27,6 +27,7 @@ KIND_BYTES = {"CALL_COPY": 32, "VOID": 0, "ERAS": 32, "ANNI": 48, "COMM": 112,
 NO_READBACK = 1
 FENCE_RELEASE = 2
 PROFILE = 4
+LOCAL_NET = 8
 
 # Scheduler (include/hvm_hip.h)
 SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
@@ -56,6 +57,7 @@ class _Out(C.Structure):
                 ("kernel_ms", C.c_double), ("eval_ms", C.c_double), ("wall_s", C.c_double),
                 ("turns", C.c_uint32), ("normalizes", C.c_uint32), ("err", C.c_uint32),
                 ("launches", C.c_uint32), ("launch_ms", C.c_double), ("waves", C.c_uint32),
+                ("saves", C.c_uint64), ("local_left", C.c_uint64),
                 ("phase_ms", C.c_double * 6),
                 ("result", C.c_void_p), ("canonical", C.c_void_p)]
 
@@ -179,6 +181,7 @@ def _result(rc: int, o: _Out) -> RunResult:
                   launches=o.launches, launch_ms=o.launch_ms,
                   diag={"steps": o.steps, "spills": o.spills, "defers": o.defers,
                         "retries": o.retries, "probes": o.probes, "waves": o.waves,
+                        "saves": o.saves, "local_left": o.local_left,
                         "phase_ms": [round(o.phase_ms[k], 3) for k in range(5)]})
     r.result = _take_str(o.result)
     r.canonical = _take_str(o.canonical)
