@@ -13,9 +13,10 @@
 #ifndef HVM_DEQ
 #define HVM_DEQ         32     // per-lane redex deque entries (power of two)
 #endif
-// minimum resident waves per SIMD the turn kernel is compiled for
+// resident waves per SIMD the HBM turn kernel is compiled for (its registers
+// are capped so that this many fit; LDS allows 7 waves per CU)
 #ifndef HVM_TURN_OCC
-#define HVM_TURN_OCC    1
+#define HVM_TURN_OCC    2
 #endif
 #define HVM_TBL_NODE    512    // per-wave node-slot table (one step's allocations)
 #define HVM_TBL_VARS    512    // per-wave var-slot table

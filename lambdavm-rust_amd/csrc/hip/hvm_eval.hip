@@ -446,6 +446,25 @@ enum {
   HVM_EMIT = 10,  // end of turn: a redex of HBM ports goes to the output bag as is
 };
 
+// A def template's port in a fresh instance: node and var indices of the
+// template become the slots allocated for this CALL (src/hvm.c:351-364).
+__device__ __forceinline__ u32 adjust_port(u32 p, const u32* tn, const u32* tv) {
+  if (hvm_is_nod(p)) return hvm_new_port(hvm_get_tag(p), tn[hvm_get_val(p)]);
+  if (hvm_is_var(p)) return hvm_new_port(HVM_VAR, tv[hvm_get_val(p)]);
+  return p;
+}
+__device__ __forceinline__ u64 adjust_pair(u64 p, const u32* tn, const u32* tv) {
+  return hvm_new_pair(adjust_port(hvm_get_fst(p), tn, tv), adjust_port(hvm_get_snd(p), tn, tv));
+}
+
+// Template pairs of a CALL loaded ahead of use (the rest as they are needed).
+#ifndef HVM_PRE_NODE
+#define HVM_PRE_NODE 8
+#endif
+#ifndef HVM_PRE_RBAG
+#define HVM_PRE_RBAG 4
+#endif
+
 // Steps a wave may spend draining its local net before giving up.
 #define HVM_EVAC_CAP (1u << 16)
 
@@ -671,6 +690,18 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
                           (LOC && rule == HVM_LEAK));
     u64 NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
     u64 NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
+    // A CALL's template: its first node and redex pairs are loaded now, so
+    // that their round trip overlaps the takes and the allocation.
+    u64 tpre[HVM_PRE_NODE], rpre[HVM_PRE_RBAG];
+    {
+      const bool inst = run && rule == HVM_CALL && !copy;
+#pragma unroll
+      for (int i = 0; i < HVM_PRE_NODE; ++i)
+        tpre[i] = (inst && (u32)i < d.nlen) ? A.pairs[d.node_off + i] : 0ull;
+#pragma unroll
+      for (int i = 0; i < HVM_PRE_RBAG; ++i)
+        rpre[i] = (inst && (u32)i < d.rlen) ? A.pairs[d.rbag_off + i] : 0ull;
+    }
 
     // ---- P2: allocation (wave-cooperative, nodes and vars together) ----
     if (LOC) {
@@ -748,15 +779,11 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     if (run) {
       if (rule == HVM_CALL && !copy) {
         for (u32 i = 0; i < d.vlen; ++i) var_put<LOC>(A, N, tv[i], HVM_NONE);
-        for (u32 i = 0; i < d.nlen; ++i) {
-          u64 p = A.pairs[d.node_off + i];
-          u32 p1 = hvm_get_fst(p), p2 = hvm_get_snd(p);
-          if (hvm_is_nod(p1)) p1 = hvm_new_port(hvm_get_tag(p1), tn[hvm_get_val(p1)]);
-          else if (hvm_is_var(p1)) p1 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p1)]);
-          if (hvm_is_nod(p2)) p2 = hvm_new_port(hvm_get_tag(p2), tn[hvm_get_val(p2)]);
-          else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
-          net_put<LOC>(A, N, tn[i], hvm_new_pair(p1, p2));
-        }
+#pragma unroll
+        for (int i = 0; i < HVM_PRE_NODE; ++i)
+          if ((u32)i < d.nlen) net_put<LOC>(A, N, tn[i], adjust_pair(tpre[i], tn, tv));
+        for (u32 i = HVM_PRE_NODE; i < d.nlen; ++i)
+          net_put<LOC>(A, N, tn[i], adjust_pair(A.pairs[d.node_off + i], tn, tv));
       } else if (rule == HVM_COMM) {
         var_put<LOC>(A, N, tv[0], HVM_NONE);
         var_put<LOC>(A, N, tv[1], HVM_NONE);
@@ -814,23 +841,25 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
             pa[0] = a; pb[0] = B1; pa[1] = a; pb[1] = B2; np = 2;
           } else {
             // the def's redexes, then its root against the caller's port,
-            // HVM_LB links at a time
-            for (u32 i = 0; i <= d.rlen; ++i) {
-              u32 p1, p2;
+            // HVM_LB links at a time (the first ones were loaded ahead)
+            static_assert(HVM_PRE_RBAG == HVM_LB, "one batch of redexes is loaded ahead");
+#pragma unroll
+            for (int k = 0; k < HVM_PRE_RBAG; ++k) {
+              if ((u32)k < d.rlen) {
+                u64 p = adjust_pair(rpre[k], tn, tv);
+                pa[k] = hvm_get_fst(p); pb[k] = hvm_get_snd(p);
+              }
+            }
+            np = min(d.rlen, (u32)HVM_PRE_RBAG);
+            for (u32 i = np;; ++i) {
+              if (np == HVM_LB) { link_batch<LOC>(L, N, ln, A, pa, pb, np); np = 0; }
               if (i < d.rlen) {
-                u64 p = A.pairs[d.rbag_off + i];
-                p1 = hvm_get_fst(p); p2 = hvm_get_snd(p);
+                u64 p = adjust_pair(A.pairs[d.rbag_off + i], tn, tv);
+                pa[np] = hvm_get_fst(p); pb[np] = hvm_get_snd(p); np++;
               } else {
-                p1 = d.root; p2 = b;
+                pa[np] = adjust_port(d.root, tn, tv); pb[np] = b; np++;
+                break;
               }
-              if (hvm_is_nod(p1)) p1 = hvm_new_port(hvm_get_tag(p1), tn[hvm_get_val(p1)]);
-              else if (hvm_is_var(p1)) p1 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p1)]);
-              if (i < d.rlen) {
-                if (hvm_is_nod(p2)) p2 = hvm_new_port(hvm_get_tag(p2), tn[hvm_get_val(p2)]);
-                else if (hvm_is_var(p2)) p2 = hvm_new_port(HVM_VAR, tv[hvm_get_val(p2)]);
-              }
-              pa[np] = p1; pb[np] = p2; np++;
-              if (np == HVM_LB && i < d.rlen) { link_batch<LOC>(L, N, ln, A, pa, pb, np); np = 0; }
             }
           }
           break;
@@ -1009,7 +1038,8 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
 }
 
 // The local-net evaluator: the same turn with node/var allocation in LDS.
-__global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_local_kernel(HvmTurnArgs A) {
+// Its LDS allows one wave per SIMD, so registers are not capped further.
+__global__ __launch_bounds__(HVM_WAVE, 1) void hvm_turn_local_kernel(HvmTurnArgs A) {
   __shared__ WaveLds L;
   __shared__ LocalNet N;
   turn_body<true>(A, L, &N);
