@@ -94,7 +94,7 @@ typedef struct hvm_hip_out {
                                     turn ends (unreachable garbage)           */
   double phase_ms[6];            /* with HVM_HIP_PROFILE: wave-time per step phase
                                     (refill+share, classify, takes+alloc,
-                                    create+release, links+count, unused)     */
+                                    creation, links+count, release wait)     */
   char* result;                  /* "Result:" text as the reference prints it */
   char* canonical;               /* result with canonical var names (tests/run.rs:112-146) */
 } hvm_hip_out;

@@ -76,7 +76,7 @@ struct HvmDevStats {
   unsigned long long probes;      // 64-slot allocator windows scanned
   // wall-clock ticks (100 MHz) spent per step phase, summed over waves, when
   // the evaluator runs with HVM_TURN_PROFILE: 0 refill+share, 1 classify+
-  // reserve, 2 takes+allocation, 3 creation+release, 4 links, 5 counting
+  // reserve, 2 takes+allocation, 3 creation, 4 links+counting, 5 release wait
   unsigned long long phase[6];
   unsigned long long leaks;       // LEAK interactions (local node copied to HBM), not in itrs
   unsigned long long saves;       // redexes moved out of a local net at the end of a turn

@@ -37,25 +37,31 @@ typedef uint32_t u32;
 __device__ __forceinline__ u32 vidx(u32 val) { return val == HVM_ROOT_VAL ? 0u : val; }
 
 // Every access is bounds-checked against the net's capacity: a port that
-// points outside the net raises HVM_ERR_BAD_PORT instead of touching memory
-// the net does not own.
+// points outside the net raises HVM_ERR_BAD_PORT, and the access goes to the
+// spare slot at index cap (each buffer has one past its capacity), never to
+// memory the net does not own.  Redirecting instead of branching around the
+// access keeps one instruction per access, so that independent accesses
+// stay in flight together.
+__device__ __forceinline__ u32 node_slot(const HvmTurnArgs& A, u32 loc) {
+  if (loc >= A.node_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); loc = A.node_cap; }
+  return loc;
+}
+__device__ __forceinline__ u32 vars_slot(const HvmTurnArgs& A, u32 val) {
+  u32 i = vidx(val);
+  if (i >= A.vars_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); i = A.vars_cap; }
+  return i;
+}
 __device__ __forceinline__ u64 node_take(const HvmTurnArgs& A, u32 loc) {
-  if (loc >= A.node_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return 0; }
-  return __hip_atomic_exchange(&A.node[loc], 0ull, __ATOMIC_RELAXED, AGENT);
+  return __hip_atomic_exchange(&A.node[node_slot(A, loc)], 0ull, __ATOMIC_RELAXED, AGENT);
 }
 __device__ __forceinline__ void node_put(const HvmTurnArgs& A, u32 loc, u64 v) {
-  if (loc >= A.node_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return; }
-  __hip_atomic_store(&A.node[loc], v, __ATOMIC_RELAXED, AGENT);
+  __hip_atomic_store(&A.node[node_slot(A, loc)], v, __ATOMIC_RELAXED, AGENT);
 }
 __device__ __forceinline__ u32 vars_xchg(const HvmTurnArgs& A, u32 val, u32 x) {
-  u32 i = vidx(val);
-  if (i >= A.vars_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return HVM_NONE; }
-  return __hip_atomic_exchange(&A.vars[i], x, __ATOMIC_RELAXED, AGENT);
+  return __hip_atomic_exchange(&A.vars[vars_slot(A, val)], x, __ATOMIC_RELAXED, AGENT);
 }
 __device__ __forceinline__ void vars_put(const HvmTurnArgs& A, u32 val, u32 x) {
-  u32 i = vidx(val);
-  if (i >= A.vars_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); return; }
-  __hip_atomic_store(&A.vars[i], x, __ATOMIC_RELAXED, AGENT);
+  __hip_atomic_store(&A.vars[vars_slot(A, val)], x, __ATOMIC_RELAXED, AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -111,10 +117,14 @@ struct WaveLds {
 
 struct Lane {
   u32 bot, top;                 // deque ring counters
-  u32 kind[HVM_K_KINDS];
   u64 bytes;
   u32 spills;
-  u32 leaks, saves;             // local-net evaluator only
+};
+
+// Wave-wide interaction counts (wave-uniform, kept in scalar registers).
+struct WaveCounts {
+  u64 kind[HVM_K_KINDS];
+  u64 leaks, saves;             // local-net evaluator only
 };
 
 // ---------------------------------------------------------------------------
@@ -260,6 +270,14 @@ __device__ __forceinline__ u32 enter(const HvmTurnArgs& A, LocalNet* N, u32 var)
 // them (the deque is LIFO, and that order keeps the evaluation depth-first).
 #define HVM_LB 4
 enum { LK_DONE = 0, LK_ENTER = 1, LK_PUBLISH = 2, LK_ORIENT = 3 };
+// Before publishing A ~> B the reference first follows B while it is a bound
+// var (src/hvm.c:796-808).  With 0 a link publishes at once, one exchange
+// fewer when B is an unbound var, but var chains are never shortened and grow
+// without bound on programs such as examples/stress (the link-loop guard
+// fires), so 1 is the only setting for general programs.
+#ifndef HVM_LINK_ENTER
+#define HVM_LINK_ENTER 1
+#endif
 
 // With the local net, a pair about to publish an HBM var A with a local B
 // (src/hvm.cu:1061-1072) either swaps (B is a var: the local var points to
@@ -281,7 +299,7 @@ __device__ __forceinline__ void link_batch(WaveLds& L, LocalNet* N, Lane& ln, co
           st[i] = LK_DONE;
           got |= 1u << i;
         } else {
-          st[i] = hvm_get_tag(pb[i]) == HVM_VAR ? (u32)LK_ENTER : (u32)LK_PUBLISH;
+          st[i] = (HVM_LINK_ENTER && hvm_get_tag(pb[i]) == HVM_VAR) ? (u32)LK_ENTER : (u32)LK_PUBLISH;
         }
       }
     }
@@ -291,7 +309,7 @@ __device__ __forceinline__ void link_batch(WaveLds& L, LocalNet* N, Lane& ln, co
         if (st[i] == LK_PUBLISH && !is_loc_val(hvm_get_val(pa[i])) && is_loc_port(pb[i])) {
           if (hvm_get_tag(pb[i]) == HVM_VAR) {
             u32 x = pa[i]; pa[i] = pb[i]; pb[i] = x;
-            st[i] = LK_ENTER;                       // B is now the HBM var
+            st[i] = HVM_LINK_ENTER ? (u32)LK_ENTER : (u32)LK_PUBLISH;   // B is now the HBM var
           } else {
             st[i] = LK_DONE;                        // LEAK redex
             got |= 1u << i;
@@ -299,12 +317,15 @@ __device__ __forceinline__ void link_batch(WaveLds& L, LocalNet* N, Lane& ln, co
         }
       }
     }
+    // one exchange per pair: extend B (B := NONE) or publish A ~> B (A := B)
     u32 r[HVM_LB];
 #pragma unroll
     for (int i = 0; i < HVM_LB; ++i) {
+      const bool ent = st[i] == LK_ENTER;
+      const u32 v = hvm_get_val(ent ? pb[i] : pa[i]);
+      const u32 x = ent ? (u32)HVM_NONE : pb[i];
       r[i] = 0;
-      if (st[i] == LK_ENTER) r[i] = var_xchg<LOC>(A, N, hvm_get_val(pb[i]), HVM_NONE);   // extend B
-      else if (st[i] == LK_PUBLISH) r[i] = var_xchg<LOC>(A, N, hvm_get_val(pa[i]), pb[i]);  // A ~> B
+      if (ent || st[i] == LK_PUBLISH) r[i] = var_xchg<LOC>(A, N, v, x);
     }
     busy = false;
 #pragma unroll
@@ -315,7 +336,7 @@ __device__ __forceinline__ void link_batch(WaveLds& L, LocalNet* N, Lane& ln, co
         } else {
           var_free<LOC>(A, N, hvm_get_val(pb[i]));  // we own both: free B
           pb[i] = r[i];
-          st[i] = hvm_get_tag(pb[i]) == HVM_VAR ? (u32)LK_ENTER : (u32)LK_PUBLISH;
+          st[i] = (HVM_LINK_ENTER && hvm_get_tag(pb[i]) == HVM_VAR) ? (u32)LK_ENTER : (u32)LK_PUBLISH;
         }
       } else if (st[i] == LK_PUBLISH) {
         if (r[i] == HVM_NONE) {
@@ -457,6 +478,16 @@ __device__ __forceinline__ u64 adjust_pair(u64 p, const u32* tn, const u32* tv) 
   return hvm_new_pair(adjust_port(hvm_get_fst(p), tn, tv), adjust_port(hvm_get_snd(p), tn, tv));
 }
 
+// Appends the link pair (p1, p2) at position *np of a batch.  The position
+// is selected with constant indices so that the batch stays in registers
+// (a runtime index would move it to scratch memory).
+__device__ __forceinline__ void put_link(u32* pa, u32* pb, u32& np, u32 p1, u32 p2) {
+#pragma unroll
+  for (int k = 0; k < HVM_LB; ++k)
+    if (np == (u32)k) { pa[k] = p1; pb[k] = p2; }
+  np++;
+}
+
 // Template pairs of a CALL loaded ahead of use (the rest as they are needed).
 #ifndef HVM_PRE_NODE
 #define HVM_PRE_NODE 8
@@ -484,11 +515,12 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
 
   Lane ln;
   ln.bot = ln.top = 0;
+  WaveCounts wc;
 #pragma unroll
-  for (int k = 0; k < HVM_K_KINDS; ++k) ln.kind[k] = 0;
+  for (int k = 0; k < HVM_K_KINDS; ++k) wc.kind[k] = 0;
+  wc.leaks = wc.saves = 0;
   ln.bytes = 0;
   ln.spills = 0;
-  ln.leaks = ln.saves = 0;
   u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0;
 
   if (LOC) {
@@ -818,6 +850,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       }
     }
 
+    HVM_PHASE(3);
     // ---- P5: publish creations before any link can expose them ----
     if (A.flags & HVM_TURN_WAITCNT_FENCE) {
       // Node/var creations are device-scope (write-through) stores: waiting
@@ -827,7 +860,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     }
 
-    HVM_PHASE(3);
+    HVM_PHASE(5);
     // ---- P6: links ----
     u64 emit = 0;         // SAVE / EMIT: the redex that leaves through the output bag
     if (run) {
@@ -855,9 +888,9 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
               if (np == HVM_LB) { link_batch<LOC>(L, N, ln, A, pa, pb, np); np = 0; }
               if (i < d.rlen) {
                 u64 p = adjust_pair(A.pairs[d.rbag_off + i], tn, tv);
-                pa[np] = hvm_get_fst(p); pb[np] = hvm_get_snd(p); np++;
+                put_link(pa, pb, np, hvm_get_fst(p), hvm_get_snd(p));
               } else {
-                pa[np] = adjust_port(d.root, tn, tv); pb[np] = b; np++;
+                put_link(pa, pb, np, adjust_port(d.root, tn, tv), b);
                 break;
               }
             }
@@ -886,13 +919,14 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
           break;
         default:
           if (LOC && rule == HVM_LEAK) {
-            if (is_loc_port(B1)) { pa[np] = g1; pb[np] = B1; np++; }
-            if (is_loc_port(B2)) { pa[np] = g2; pb[np] = B2; np++; }
-            pa[np] = a; pb[np] = hvm_new_port(hvm_get_tag(b), gn[0]); np++;
+            if (is_loc_port(B1)) put_link(pa, pb, np, g1, B1);
+            if (is_loc_port(B2)) put_link(pa, pb, np, g2, B2);
+            put_link(pa, pb, np, a, hvm_new_port(hvm_get_tag(b), gn[0]));
           } else if (LOC && rule == HVM_SAVE) {
-            u32 k = 0, x = a, y = b;
-            if (is_loc_port(a)) { x = hvm_new_port(HVM_VAR, gv[k]); pa[np] = x; pb[np] = a; np++; k++; }
-            if (is_loc_port(b)) { y = hvm_new_port(HVM_VAR, gv[k]); pa[np] = y; pb[np] = b; np++; k++; }
+            u32 x = a, y = b;
+            const bool la = is_loc_port(a);
+            if (la) { x = hvm_new_port(HVM_VAR, gv[0]); put_link(pa, pb, np, x, a); }
+            if (is_loc_port(b)) { y = hvm_new_port(HVM_VAR, gv[la ? 1 : 0]); put_link(pa, pb, np, y, b); }
             emit = hvm_new_pair(x, y);
           } else if (LOC && rule == HVM_EMIT) {
             emit = hvm_new_pair(a, b);
@@ -901,27 +935,36 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       }
       if (np) link_batch<LOC>(L, N, ln, A, pa, pb, np);
 
-      // ---- P7: counting ----
-      switch (rule) {
-        case HVM_CALL:
-          if (copy) { ln.kind[HVM_K_CALL_COPY]++; ln.bytes += 32; }
-          else { ln.kind[HVM_K_CALL]++; ln.bytes += d.call_bytes; }
-          break;
-        case HVM_VOID: ln.kind[HVM_K_VOID]++; break;
-        case HVM_ERAS: ln.kind[HVM_K_ERAS]++; ln.bytes += 32; break;
-        case HVM_ANNI: ln.kind[HVM_K_ANNI]++; ln.bytes += 48; break;
-        case HVM_COMM: ln.kind[HVM_K_COMM]++; ln.bytes += 112; break;
-        case HVM_OPER:
-          if (oper1) { ln.kind[HVM_K_OPER1]++; ln.bytes += 32; }
-          else { ln.kind[HVM_K_OPER2]++; ln.bytes += 40; }
-          break;
-        case HVM_SWIT:
-          if (hvm_get_u24(hvm_get_val(a)) == 0) { ln.kind[HVM_K_SWIT0]++; ln.bytes += 32; }
-          else { ln.kind[HVM_K_SWITN]++; ln.bytes += 40; }
-          break;
-        case HVM_LEAK: ln.leaks++; break;
-        case HVM_SAVE: ln.saves++; break;
-        default: break;
+    }
+    // ---- P7: counting (ITRS += 1 per non-LINK rule, src/hvm.c:1116-1118) ----
+    {
+      u32 kk = HVM_K_KINDS, kb = 0;   // kind slot and canonical bytes of this lane's interaction
+      if (run) {
+        switch (rule) {
+          case HVM_CALL:
+            if (copy) { kk = HVM_K_CALL_COPY; kb = 32; }
+            else { kk = HVM_K_CALL; kb = d.call_bytes; }
+            break;
+          case HVM_VOID: kk = HVM_K_VOID; break;
+          case HVM_ERAS: kk = HVM_K_ERAS; kb = 32; break;
+          case HVM_ANNI: kk = HVM_K_ANNI; kb = 48; break;
+          case HVM_COMM: kk = HVM_K_COMM; kb = 112; break;
+          case HVM_OPER:
+            kk = oper1 ? HVM_K_OPER1 : HVM_K_OPER2; kb = oper1 ? 32 : 40;
+            break;
+          case HVM_SWIT:
+            if (hvm_get_u24(hvm_get_val(a)) == 0) { kk = HVM_K_SWIT0; kb = 32; }
+            else { kk = HVM_K_SWITN; kb = 40; }
+            break;
+          default: break;
+        }
+      }
+      ln.bytes += kb;
+#pragma unroll
+      for (int k = 0; k < HVM_K_KINDS; ++k) wc.kind[k] += __popcll(__ballot(kk == (u32)k));
+      if (LOC) {
+        wc.leaks += __popcll(__ballot(run && rule == HVM_LEAK));
+        wc.saves += __popcll(__ballot(run && rule == HVM_SAVE));
       }
     }
     if (LOC && evac) {
@@ -987,19 +1030,15 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
 
   // ---- statistics: one atomic per counter per wave ----
   {
-    u64 itrs = 0;
+    u64 s_itrs = 0;
+    u64 kinds[HVM_K_KINDS];
 #pragma unroll
-    for (int k = 0; k < HVM_K_KINDS; ++k) itrs += ln.kind[k];
-    u64 s_itrs = wave_sum64(itrs);
+    for (int k = 0; k < HVM_K_KINDS; ++k) { kinds[k] = wc.kind[k]; s_itrs += wc.kind[k]; }
     u64 s_bytes = wave_sum64(ln.bytes);
     u64 s_spill = wave_sum64(ln.spills);
     defers = wave_sum64(defers);    // per-lane counts; steps/probes/lane_steps are wave-uniform
     retries = wave_sum64(retries);
-    u64 s_leaks = 0, s_saves = 0;
-    if (LOC) { s_leaks = wave_sum64(ln.leaks); s_saves = wave_sum64(ln.saves); }
-    u64 kinds[HVM_K_KINDS];
-#pragma unroll
-    for (int k = 0; k < HVM_K_KINDS; ++k) kinds[k] = wave_sum64(ln.kind[k]);
+    u64 s_leaks = wc.leaks, s_saves = wc.saves;
     if (lane == 0) {
       if (s_itrs) atomicAdd(&A.stats->itrs, s_itrs);
       if (s_bytes) atomicAdd(&A.stats->bytes, s_bytes);

@@ -125,8 +125,10 @@ struct hvm_hip_ctx {
     if (node_slice < 4 * HVM_WAVE || vars_slice < 4 * HVM_WAVE) return fail("net too small for the wave count");
     if (!ck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
     if (!ck(hipEventCreate(&ev0), "hipEventCreate") || !ck(hipEventCreate(&ev1), "hipEventCreate")) return false;
-    if (!ck(hipMalloc(&node, (size_t)node_cap * 8), "hipMalloc(node)")) return false;
-    if (!ck(hipMalloc(&vars, (size_t)vars_cap * 4), "hipMalloc(vars)")) return false;
+    // one spare slot past each capacity: the evaluator's accesses through an
+    // out-of-range port land there (and raise HVM_ERR_BAD_PORT)
+    if (!ck(hipMalloc(&node, ((size_t)node_cap + 1) * 8), "hipMalloc(node)")) return false;
+    if (!ck(hipMalloc(&vars, ((size_t)vars_cap + 1) * 4), "hipMalloc(vars)")) return false;
     if (!ck(hipMalloc(&bag[0], (size_t)bag_cap * 8), "hipMalloc(bag)")) return false;
     if (!ck(hipMalloc(&bag[1], (size_t)bag_cap * 8), "hipMalloc(bag)")) return false;
     if (!ck(hipMalloc(&ctl, sizeof(HvmCtl)), "hipMalloc(ctl)")) return false;

@@ -182,7 +182,7 @@ def _result(rc: int, o: _Out) -> RunResult:
                   diag={"steps": o.steps, "spills": o.spills, "defers": o.defers,
                         "retries": o.retries, "probes": o.probes, "waves": o.waves,
                         "saves": o.saves, "local_left": o.local_left,
-                        "phase_ms": [round(o.phase_ms[k], 3) for k in range(5)]})
+                        "phase_ms": [round(o.phase_ms[k], 3) for k in range(6)]})
     r.result = _take_str(o.result)
     r.canonical = _take_str(o.canonical)
     return r
