@@ -37,20 +37,13 @@ typedef uint32_t u32;
 __device__ __forceinline__ u32 vidx(u32 val) { return val == HVM_ROOT_VAL ? 0u : val; }
 
 // Every access is bounds-checked against the net's capacity: a port that
-// points outside the net raises HVM_ERR_BAD_PORT, and the access goes to the
-// spare slot at index cap (each buffer has one past its capacity), never to
-// memory the net does not own.  Redirecting instead of branching around the
-// access keeps one instruction per access, so that independent accesses
-// stay in flight together.
-__device__ __forceinline__ u32 node_slot(const HvmTurnArgs& A, u32 loc) {
-  if (loc >= A.node_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); loc = A.node_cap; }
-  return loc;
-}
-__device__ __forceinline__ u32 vars_slot(const HvmTurnArgs& A, u32 val) {
-  u32 i = vidx(val);
-  if (i >= A.vars_cap) { atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT); i = A.vars_cap; }
-  return i;
-}
+// points outside the net reaches the spare slot at index cap (each buffer has
+// one past its capacity), never memory the net does not own.  The check is a
+// branch-free clamp, so that independent accesses stay in flight together;
+// hvm_between_kernel raises HVM_ERR_BAD_PORT once a spare slot was written,
+// and a take through such a port raises it on the retry path.
+__device__ __forceinline__ u32 node_slot(const HvmTurnArgs& A, u32 loc) { return min(loc, A.node_cap); }
+__device__ __forceinline__ u32 vars_slot(const HvmTurnArgs& A, u32 val) { return min(vidx(val), A.vars_cap); }
 __device__ __forceinline__ u64 node_take(const HvmTurnArgs& A, u32 loc) {
   return __hip_atomic_exchange(&A.node[node_slot(A, loc)], 0ull, __ATOMIC_RELAXED, AGENT);
 }
@@ -155,16 +148,9 @@ __device__ __forceinline__ bool is_loc_port(u32 p) {
   u32 t = hvm_get_tag(p);
   return (t == HVM_VAR || t >= HVM_CON) && is_loc_val(hvm_get_val(p));
 }
-__device__ __forceinline__ u32 loc_n(const HvmTurnArgs& A, u32 v) {
-  u32 i = v - HVM_LOCAL_BASE;
-  if (i >= HVM_LNODE) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT);
-  return i & (HVM_LNODE - 1);
-}
-__device__ __forceinline__ u32 loc_v(const HvmTurnArgs& A, u32 v) {
-  u32 i = v - HVM_LOCAL_BASE;
-  if (i >= HVM_LVARS) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT);
-  return i & (HVM_LVARS - 1);
-}
+// Local slot of a local port (masked: always inside the LDS buffers).
+__device__ __forceinline__ u32 loc_n(const HvmTurnArgs& A, u32 v) { return (v - HVM_LOCAL_BASE) & (HVM_LNODE - 1); }
+__device__ __forceinline__ u32 loc_v(const HvmTurnArgs& A, u32 v) { return (v - HVM_LOCAL_BASE) & (HVM_LVARS - 1); }
 
 // Net accessors of the evaluator: local slots in LDS (LOC only), else HBM.
 template <bool LOC>
@@ -469,10 +455,12 @@ enum {
 
 // A def template's port in a fresh instance: node and var indices of the
 // template become the slots allocated for this CALL (src/hvm.c:351-364).
+// Branch-free: one table read per port (slot 0 of the var table for ports
+// that are neither), so that the reads of several ports go out together.
 __device__ __forceinline__ u32 adjust_port(u32 p, const u32* tn, const u32* tv) {
-  if (hvm_is_nod(p)) return hvm_new_port(hvm_get_tag(p), tn[hvm_get_val(p)]);
-  if (hvm_is_var(p)) return hvm_new_port(HVM_VAR, tv[hvm_get_val(p)]);
-  return p;
+  const bool nod = hvm_is_nod(p), var = hvm_is_var(p);
+  const u32 x = (nod ? tn : tv)[(nod || var) ? hvm_get_val(p) : 0u];
+  return (nod || var) ? hvm_new_port(hvm_get_tag(p), x) : p;
 }
 __device__ __forceinline__ u64 adjust_pair(u64 p, const u32* tn, const u32* tv) {
   return hvm_new_pair(adjust_port(hvm_get_fst(p), tn, tv), adjust_port(hvm_get_snd(p), tn, tv));
@@ -776,6 +764,9 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     if ((take_a && NA == 0) || (take_b && NB == 0)) {
       // A node this redex points to is not visible yet: put everything back
       // and retry the redex on a later step (src/hvm.c:1112-1114).
+      if ((take_a && NA == 0 && !is_loc_val(hvm_get_val(a)) && hvm_get_val(a) >= A.node_cap) ||
+          (take_b && NB == 0 && !is_loc_val(hvm_get_val(b)) && hvm_get_val(b) >= A.node_cap))
+        atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT);   // not a node of this net
       if (take_a && NA) net_put<LOC>(A, N, hvm_get_val(a), NA);
       if (take_b && NB) net_put<LOC>(A, N, hvm_get_val(b), NB);
       // slots allocated for this interaction go back (they may be pool claims)
@@ -810,10 +801,31 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     u32 g1 = 0, g2 = 0;   // LEAK: HBM stand-ins for the node's ports
     if (run) {
       if (rule == HVM_CALL && !copy) {
-        for (u32 i = 0; i < d.vlen; ++i) var_put<LOC>(A, N, tv[i], HVM_NONE);
+        // Table entries are read unconditionally, a chunk at a time, before
+        // the stores that need them (entries past the def's own lie inside
+        // the table: offsets are at most HVM_TBL_NODE/VARS, the tables have
+        // HVM_TBL_X more entries).
+        static_assert(HVM_TBL_X >= 8 && HVM_PRE_NODE % 4 == 0, "table reads stay inside the tables");
 #pragma unroll
-        for (int i = 0; i < HVM_PRE_NODE; ++i)
-          if ((u32)i < d.nlen) net_put<LOC>(A, N, tn[i], adjust_pair(tpre[i], tn, tv));
+        for (int c = 0; c < 8; c += 4) {
+          u32 vs[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) vs[k] = tv[c + k];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if ((u32)(c + k) < d.vlen) var_put<LOC>(A, N, vs[k], HVM_NONE);
+        }
+        for (u32 i = 8; i < d.vlen; ++i) var_put<LOC>(A, N, tv[i], HVM_NONE);
+#pragma unroll
+        for (int c = 0; c < HVM_PRE_NODE; c += 4) {
+          u32 locs[4];
+          u64 vals[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) { locs[k] = tn[c + k]; vals[k] = adjust_pair(tpre[c + k], tn, tv); }
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if ((u32)(c + k) < d.nlen) net_put<LOC>(A, N, locs[k], vals[k]);
+        }
         for (u32 i = HVM_PRE_NODE; i < d.nlen; ++i)
           net_put<LOC>(A, N, tn[i], adjust_pair(A.pairs[d.node_off + i], tn, tv));
       } else if (rule == HVM_COMM) {
@@ -1190,6 +1202,11 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
     if (!c->done) {
       u32 n = c->n_out;
       c->turns += 1;
+      // the spare slots past the capacities are written only through ports
+      // that point outside the net
+      if (__hip_atomic_load(&A.node[A.node_cap], __ATOMIC_RELAXED, AGENT) != 0 ||
+          __hip_atomic_load(&A.vars[A.vars_cap], __ATOMIC_RELAXED, AGENT) != 0)
+        c->err |= HVM_ERR_BAD_PORT;
       if (c->err || n > A.bag_cap) {
         if (n > A.bag_cap) c->err |= HVM_ERR_BAG_OVERFLOW;
         c->done = 1;

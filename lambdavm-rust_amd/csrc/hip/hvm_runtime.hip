@@ -243,8 +243,8 @@ struct hvm_hip_ctx {
   }
 
   bool reset_heap() {
-    if (!ck(hipMemsetAsync(node, 0, (size_t)node_cap * 8, stream), "clear node")) return false;
-    if (!ck(hipMemsetAsync(vars, 0, (size_t)vars_cap * 4, stream), "clear vars")) return false;
+    if (!ck(hipMemsetAsync(node, 0, ((size_t)node_cap + 1) * 8, stream), "clear node")) return false;
+    if (!ck(hipMemsetAsync(vars, 0, ((size_t)vars_cap + 1) * 4, stream), "clear vars")) return false;
     if (!ck(hipMemsetAsync(wave_state, 0, nwaves * sizeof(HvmWaveState), stream), "clear cursors")) return false;
     if (!ck(hipMemsetAsync(ctl, 0, sizeof(HvmCtl), stream), "clear ctl")) return false;
     return ck(hipStreamSynchronize(stream), "reset sync");
