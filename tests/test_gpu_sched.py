@@ -83,13 +83,24 @@ def test_scheduler_never_changes_results(hip):
     assert len(seen) == 1
 
 
-def test_batch_matches_per_net_reference(hip):
+def _tuned_batch_cfg():
+    path = os.path.join(ROOT, "lambdavm-rust_amd", "tuned_batch.json")
+    return json.load(open(path))["config"] if os.path.exists(path) else None
+
+
+@pytest.mark.parametrize("which", ["default", "tuned"])
+def test_batch_matches_per_net_reference(hip, which):
     g = json.load(open(os.path.join(ROOT, "tests", "golden", "batch.json")))
     code, nets = P.batch(g["seed"], g["n"])
     fids = P.def_fids(code)
     entries = [fids[f"net{i}"] for i in range(len(nets))]
     est = [P.batch_estimate(gen, params) for gen, params, _, _ in nets]
-    with hip.Context(time_limit_s=300, node_cap=1 << 29, vars_cap=1 << 29) as ctx:
+    cfg = {"node_cap": 1 << 29, "vars_cap": 1 << 29}
+    if which == "tuned":
+        if _tuned_batch_cfg() is None:
+            pytest.skip("no tuned configuration committed")
+        cfg = _tuned_batch_cfg()
+    with hip.Context(time_limit_s=300, **cfg) as ctx:
         ctx.load(hip.compile_book(code))
         per, total = ctx.eval_batch(entries, est)
     assert total.itrs == g["total_itrs"]
@@ -122,3 +133,17 @@ def test_trained_scheduler_gain(hip):
     plain, learned = np.median(ms[hip.SCHED_PLAIN]), np.median(ms[hip.SCHED_LEARNED])
     print(f"church(12,12) median eval: plain {plain:.2f} ms, learned {learned:.2f} ms, gain {plain / learned:.3f}x")
     assert learned <= plain * 1.05
+
+
+def test_autotuner_keeps_results(hip):
+    """The autotuner's search on a small batch: every candidate reproduces the
+    default's ITRS (it raises otherwise) and the chosen configuration is one
+    of the measured ones."""
+    import autotune as T
+    w = T.batched(seed=5, n=12)
+    rep = T.tune(w, repeats=1, log=print, base_cfg={},
+                 space=[("waves", [512, 0]), ("idle_pct", [101, 0]), ("turn_steps", [96, 0]),
+                        ("flags", [hip.LOCAL_NET, 0])])
+    assert rep["itrs"] > 0
+    assert rep["tuned_ms"] <= rep["default_ms"]
+    assert any(m.get("cfg") == rep["config"] for m in rep["measurements"])
