@@ -48,6 +48,41 @@ def cpu_baseline(words, itrs_expected):
             "sample": "one full run of the bench workload by the single-thread C restatement (oracle/)"}
 
 
+def profile_values(k):
+    """key: value lines of the committed counter summary for this workload."""
+    path = os.path.join(ROOT, "profiles", f"hvm_turn_kernel.bitonic{k}.md")
+    vals = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.strip().strip("`").split(":")
+            if len(parts) == 2 and parts[0].strip().isidentifier():
+                try:
+                    vals[parts[0].strip()] = float(parts[1])
+                except ValueError:
+                    pass
+    return vals, os.path.relpath(path, ROOT)
+
+
+def atomic_roofline(k, itrs_per_s):
+    """The evaluator's exchange rate against the measured device-scope
+    exchange rate on random slots (hvm_hip_atomic_rate): a 64 MiB buffer (fits
+    the 256 MiB Infinity Cache, as the bench net's live part does) and a
+    1 GiB one (HBM)."""
+    import hvm_hip
+    vals, src = profile_values(k)
+    per_itr = vals.get("TCC_ATOMIC_per_interaction")
+    peaks = {f"{b}B_{name}": hvm_hip.atomic_rate(slots, b) / 1e9
+             for b in (4, 8) for name, slots in (("64MiB", (64 << 20) // b), ("1GiB", (1 << 30) // b))}
+    peak = max(peaks["4B_64MiB"], peaks["8B_64MiB"])
+    out = {"unit": "Gatomic/s", "peak": round(peak, 2),
+           "peaks_measured": {k2: round(v, 2) for k2, v in peaks.items()}}
+    if per_itr:
+        achieved = itrs_per_s * per_itr / 1e9
+        out.update({"achieved": round(achieved, 2), "frac": round(achieved / peak, 4),
+                    "atomics_per_interaction": per_itr, "source": src})
+    return out
+
+
 def measured_traffic(k):
     """HBM bytes per turn-kernel dispatch for this workload, from the committed
     rocprofv3 counter summary: FETCH_SIZE and WRITE_SIZE (KiB, summed over the
@@ -197,6 +232,7 @@ def main():
         "schedulers": compare,
     }
     compare[args.sched + "_mips"] = out["value"]
+    out["atomic_roofline"] = atomic_roofline(args.k, itrs * args.steps / (kernel_ms * 1e-3))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(words, itrs)
     print(json.dumps(out), flush=True)

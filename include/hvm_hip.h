@@ -183,6 +183,16 @@ int hvm_mlp_train(const float* params_in, const float* x, const uint32_t* action
                   float* loss);
 
 /* ------------------------------------------------------------------------ */
+/* Measurement (no counterpart in the reference)                            */
+/* ------------------------------------------------------------------------ */
+
+/* The atomic roofline: device-scope exchanges per second on pseudo-random
+ * slots of a fresh buffer of `slots` (a power of two, 2^10..2^31) elements of
+ * `bytes` (4: var-sized, 8: node-sized), every lane keeping 8 exchanges in
+ * flight.  The evaluator's takes and links are such exchanges. */
+int hvm_hip_atomic_rate(uint64_t slots, uint32_t bytes, double* ops_per_s);
+
+/* ------------------------------------------------------------------------ */
 /* Front end (the reference's Rust host, restated in C++)                   */
 /* ------------------------------------------------------------------------ */
 

@@ -1,0 +1,79 @@
+// hvm_probe.hip — measurement of the atomic roofline of the evaluator: the
+// rate at which the device performs device-scope exchanges on random slots of
+// a net-sized buffer, the operation the evaluator's takes and links consist
+// of.  Used by bench.py to report the evaluator's exchange rate against this
+// ceiling next to the HBM roofline.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/hvm_hip.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t xorshift32(uint32_t x) {
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  return x;
+}
+
+// Every lane keeps HVM_PROBE_INFLIGHT exchanges in flight per round, on
+// pseudo-random slots of buf[0, mask]; the results are folded into a value the
+// kernel writes only if it hits an impossible pattern, so no exchange is dead.
+constexpr int HVM_PROBE_INFLIGHT = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void atomic_probe_kernel(T* buf, uint32_t mask, uint32_t rounds, T* sink) {
+  uint32_t x = (blockIdx.x * blockDim.x + threadIdx.x) * 2654435761u + 0x9E3779B9u;
+  x = x ? x : 1u;
+  T acc = 0;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    T v[HVM_PROBE_INFLIGHT];
+#pragma unroll
+    for (int k = 0; k < HVM_PROBE_INFLIGHT; ++k) {
+      x = xorshift32(x);
+      v[k] = __hip_atomic_exchange(&buf[x & mask], (T)x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int k = 0; k < HVM_PROBE_INFLIGHT; ++k) acc ^= v[k];
+  }
+  if (acc == (T)0x5A5A5A5A3C3C3C3Cull) sink[0] = acc;
+}
+
+template <typename T>
+int run_probe(uint64_t slots, double* ops_per_s) {
+  if (slots < 1024 || (slots & (slots - 1)) || slots > (1ull << 31)) return HVM_HIP_E_ARG;
+  T* buf = nullptr;
+  T* sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = HVM_HIP_E_DEVICE;
+  const uint32_t threads = 256, blocks = 8192, rounds = 16;
+  if (hipMalloc(&buf, slots * sizeof(T)) != hipSuccess) return HVM_HIP_E_DEVICE;
+  if (hipMalloc(&sink, sizeof(T)) == hipSuccess && hipMemset(buf, 0, slots * sizeof(T)) == hipSuccess &&
+      hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+    const uint32_t mask = (uint32_t)(slots - 1);
+    hipLaunchKernelGGL(atomic_probe_kernel<T>, dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
+    hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(atomic_probe_kernel<T>, dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
+    hipEventRecord(e1, 0);
+    float ms = 0;
+    if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0) {
+      *ops_per_s = (double)threads * blocks * rounds * HVM_PROBE_INFLIGHT / (ms * 1e-3);
+      rc = HVM_HIP_OK;
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (sink) hipFree(sink);
+  hipFree(buf);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int hvm_hip_atomic_rate(uint64_t slots, uint32_t bytes, double* ops_per_s) {
+  if (!ops_per_s) return HVM_HIP_E_ARG;
+  if (bytes == 4) return run_probe<uint32_t>(slots, ops_per_s);
+  if (bytes == 8) return run_probe<unsigned long long>(slots, ops_per_s);
+  return HVM_HIP_E_ARG;
+}

@@ -302,6 +302,17 @@ def _farr(xs):
     return (C.c_float * len(xs))(*xs)
 
 
+def atomic_rate(slots: int, nbytes: int) -> float:
+    """Device-scope exchanges per second on random slots (hvm_hip_atomic_rate)."""
+    L = lib()
+    L.hvm_hip_atomic_rate.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
+    out = C.c_double(0)
+    rc = L.hvm_hip_atomic_rate(slots, nbytes, C.byref(out))
+    if rc != 0:
+        raise HvmError(f"hvm_hip_atomic_rate failed: {STATUS.get(rc, rc)}")
+    return out.value
+
+
 def mlp_forward(params_fp16, x_rows):
     """x_rows: list of 32-float rows → list of 4-float predictions."""
     n = len(x_rows)
