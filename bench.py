@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--k", type=int, default=16, help="log2 of the key count")
     ap.add_argument("--sched", choices=("plain", "learned"), default="plain",
                     help="turn-length scheduler of the timed steps (the other one is timed after them)")
+    ap.add_argument("--no-sched-compare", action="store_true",
+                    help="do not time the other scheduler (profiling runs: only the bench's own evaluations)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (torch), to calibrate "
@@ -180,7 +182,7 @@ def main():
     # the other scheduler on the same workload, after the timed region
     other = "plain" if args.sched == "learned" else "learned"
     compare = {args.sched + "_mips": None, other + "_mips": None}
-    if weights is not None:
+    if weights is not None and not args.no_sched_compare:
         use(ctx, other)
         ctx.eval()
         t2 = time.perf_counter()
