@@ -156,16 +156,17 @@ def main():
               "sorted_keys_match": P.flatten_tree(P.parse_tree_text(r0.result)) == sorted_keys}
     ctx.close()
 
-    weights = None
+    weights, margin = None, 0
     wpath = os.path.join(ROOT, "lambdavm-rust_amd", "sched_weights.json")
     if os.path.exists(wpath):
-        weights = json.load(open(wpath))["params_fp16"]
+        wj = json.load(open(wpath))
+        weights, margin = wj["params_fp16"], wj.get("margin_pct", 0)
     if args.sched == "learned" and weights is None:
         sys.exit("bench.py: --sched learned needs lambdavm-rust_amd/sched_weights.json")
 
     def use(ctx, sched):
         if sched == "learned":
-            ctx.set_scheduler(hvm_hip.SCHED_LEARNED, params_fp16=weights)
+            ctx.set_scheduler(hvm_hip.SCHED_LEARNED, explore_pct=margin, params_fp16=weights)
         else:
             ctx.set_scheduler(hvm_hip.SCHED_PLAIN)
 

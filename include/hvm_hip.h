@@ -152,7 +152,10 @@ int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_net
  * interaction does, so normal form and ITRS do not depend on it.
  *   HVM_HIP_SCHED_PLAIN   fixed heuristic on the number of pending redexes
  *   HVM_HIP_SCHED_LEARNED argmax of a 32->64->64->4 MLP (fp16 params,
- *                         HVM_HIP_MLP_PARAMS values) evaluated on the device
+ *                         HVM_HIP_MLP_PARAMS values) evaluated on the device;
+ *                         it replaces the heuristic's choice only when its
+ *                         predicted reward is higher by more than
+ *                         explore_pct/100 (the heuristic stays the fallback)
  *   HVM_HIP_SCHED_EXPLORE the heuristic with explore_pct% random choices
  *                         (seeded), for collecting training traces
  * trace_cap > 0 records one HVM_HIP_TRACE_REC-float record per decision:

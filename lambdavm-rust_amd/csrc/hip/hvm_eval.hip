@@ -1262,8 +1262,12 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
       y[t] = acc;
     }
     __syncthreads();
-    action = 0;
-    for (u32 k = 1; k < HVM_ACTIONS; ++k) if (y[k] > y[action]) action = k;
+    // The plain heuristic is the fallback: the network overrides its choice
+    // only when it predicts a rate higher by more than the margin
+    // (explore_pct / 100, in log2 units of interactions per µs).
+    u32 best = 0;
+    for (u32 k = 1; k < HVM_ACTIONS; ++k) if (y[k] > y[best]) best = k;
+    if (y[best] > y[action] + 0.01f * (float)s->explore_pct) action = best;
   } else if (s->mode == HVM_SCHED_EXPLORE) {
     u32 h = (s->seed * 0x9E3779B9u) ^ (c->turns * 0x85EBCA6Bu);
     h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12;

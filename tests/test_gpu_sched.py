@@ -126,7 +126,9 @@ def test_trained_scheduler_gain(hip):
         for i, seed in enumerate(range(80, 88)):
             order = (hip.SCHED_PLAIN, hip.SCHED_LEARNED) if i % 2 == 0 else (hip.SCHED_LEARNED, hip.SCHED_PLAIN)
             for mode in order:
-                ctx.set_scheduler(mode, seed=seed, params_fp16=w["params_fp16"] if mode == hip.SCHED_LEARNED else None)
+                learned = mode == hip.SCHED_LEARNED
+                ctx.set_scheduler(mode, seed=seed, explore_pct=w.get("margin_pct", 0) if learned else 0,
+                                  params_fp16=w["params_fp16"] if learned else None)
                 r = ctx.eval()
                 assert r.itrs == case["itrs"] and r.result == str(expect)
                 ms[mode].append(r.eval_ms)
