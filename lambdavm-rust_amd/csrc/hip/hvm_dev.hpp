@@ -18,10 +18,11 @@
 #ifndef HVM_TURN_OCC
 #define HVM_TURN_OCC    2
 #endif
-#define HVM_TBL_NODE    512    // per-wave node-slot table (one step's allocations)
-#define HVM_TBL_VARS    512    // per-wave var-slot table
-#define HVM_TBL_X       128    // more entries of each table: HBM slots of the
-                               // local-net evaluator's LEAK and SAVE steps
+#define HVM_TBL_NODE    256    // per-wave node-slot table (one step's allocations)
+#define HVM_TBL_VARS    256    // per-wave var-slot table
+#define HVM_TBL_SLACK   8      // entries past the tables, read ahead and never used
+#define HVM_TBL_X       128    // tables of the local-net evaluator's HBM slots
+                               // (LEAK and SAVE steps)
 
 // Local net (the local-net evaluator): each wave owns HVM_LNODE node slots and
 // HVM_LVARS var slots in LDS.  A node or var port whose value is
@@ -40,7 +41,7 @@
 
 // Largest def the evaluator instantiates in one interaction.  Defs are
 // checked at load time.
-#define HVM_GPU_DEF_MAX 512
+#define HVM_GPU_DEF_MAX 256
 
 // ROOT is var 0x1FFFFFFF; the evaluator stores it in var slot 0, which the
 // allocators never hand out (src/hvm.c:717: slot 0 is skipped).

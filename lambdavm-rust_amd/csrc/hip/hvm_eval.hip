@@ -101,8 +101,8 @@ __device__ __forceinline__ u64 wave_sum64(u64 x) {
 
 struct WaveLds {
   u64 deq[HVM_DEQ * HVM_WAVE];  // deque entry j of lane l at [j*64 + l]
-  u32 tbl_n[HVM_TBL_NODE + HVM_TBL_X];
-  u32 tbl_v[HVM_TBL_VARS + HVM_TBL_X];
+  u32 tbl_n[HVM_TBL_NODE + HVM_TBL_SLACK];
+  u32 tbl_v[HVM_TBL_VARS + HVM_TBL_SLACK];
   u64 inbox[HVM_WAVE];
   u32 target[HVM_WAVE];
   u32 cursor_n, cursor_v;       // allocator scratch
@@ -136,6 +136,7 @@ struct LocalNet {
   uint16_t vfree[HVM_LVARS];
   u32 ntop, vtop;               // free slots on each stack
   u32 sp;                       // entries on the wave's private HBM spill stack
+  u32 xtn[HVM_TBL_X], xtv[HVM_TBL_X];   // HBM slots of one step's LEAK / SAVE
 };
 static_assert((HVM_LNODE & (HVM_LNODE - 1)) == 0 && HVM_LNODE <= 65536, "HVM_LNODE: power of two <= 2^16");
 static_assert((HVM_LVARS & (HVM_LVARS - 1)) == 0 && HVM_LVARS <= 65536, "HVM_LVARS: power of two <= 2^16");
@@ -411,7 +412,7 @@ __device__ __forceinline__ bool scan_advance(Scan& s, u32 pool_lo, u32 pool_len,
 // both buffers are scanned concurrently.  All arguments are wave-uniform.
 __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u32 n_lo, u32 v_lo,
                                             HvmWaveState& cur, u32 need_n, u32 need_v, u64& probes, u32& oom,
-                                            u32 tbl_base) {
+                                            u32* tbl_n, u32* tbl_v) {
   const u32 lane = lane_id();
   Scan sn = {n_lo, A.node_slice, cur.node_cur, 0, need_n, 0, false};
   Scan sv = {v_lo, A.vars_slice, cur.vars_cur, 0, need_v, 0, false};
@@ -425,12 +426,12 @@ __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u3
     u64 vn = pn ? __hip_atomic_load(&A.node[in_], __ATOMIC_RELAXED, AGENT) : 1ull;
     u32 vv = pv ? __hip_atomic_load(&A.vars[iv], __ATOMIC_RELAXED, AGENT) : 1u;
     if (pn) {
-      scan_window(A.node, sn, vn, in_, rn, 1u, ~0ull, L.tbl_n + tbl_base, &L.cursor_n);
+      scan_window(A.node, sn, vn, in_, rn, 1u, ~0ull, tbl_n, &L.cursor_n);
       probes++;
       if (!scan_advance(sn, A.node_pool_lo, A.node_cap - A.node_pool_lo, cur.node_ovf)) { oom = HVM_ERR_OOM_NODE; return false; }
     }
     if (pv) {
-      scan_window(A.vars, sv, vv, iv, rv, A.root_vars, (u32)HVM_NONE, L.tbl_v + tbl_base, &L.cursor_v);
+      scan_window(A.vars, sv, vv, iv, rv, A.root_vars, (u32)HVM_NONE, tbl_v, &L.cursor_v);
       probes++;
       if (!scan_advance(sv, A.vars_pool_lo, A.vars_cap - A.vars_pool_lo, cur.vars_ovf)) { oom = HVM_ERR_OOM_VARS; return false; }
     }
@@ -741,7 +742,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       __syncthreads();
       if (need_xn || need_xv) {
         u32 oom = 0;
-        if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_xn, need_xv, probes, oom, HVM_TBL_NODE)) {
+        if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_xn, need_xv, probes, oom, N->xtn, N->xtv)) {
           atomicOr(&A.ctl->err, oom);
           if (run) ln.top++;
           break;
@@ -749,7 +750,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       }
     } else if (need_n || need_v) {
       u32 oom = 0;
-      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom, 0)) {
+      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom, L.tbl_n, L.tbl_v)) {
         // out of slots in the slice and the pool: the evaluation stops with an error
         atomicOr(&A.ctl->err, oom);
         if (run) ln.top++;
@@ -758,8 +759,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     }
     const u32* tn = L.tbl_n + off_n;
     const u32* tv = L.tbl_v + off_v;
-    const u32* gn = L.tbl_n + HVM_TBL_NODE + off_xn;   // HBM slots of LEAK / SAVE
-    const u32* gv = L.tbl_v + HVM_TBL_VARS + off_xv;
+    const u32* gn = LOC ? N->xtn + off_xn : tn;   // HBM slots of LEAK / SAVE
+    const u32* gv = LOC ? N->xtv + off_xv : tv;
 
     if ((take_a && NA == 0) || (take_b && NB == 0)) {
       // A node this redex points to is not visible yet: put everything back
@@ -804,8 +805,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         // Table entries are read unconditionally, a chunk at a time, before
         // the stores that need them (entries past the def's own lie inside
         // the table: offsets are at most HVM_TBL_NODE/VARS, the tables have
-        // HVM_TBL_X more entries).
-        static_assert(HVM_TBL_X >= 8 && HVM_PRE_NODE % 4 == 0, "table reads stay inside the tables");
+        // HVM_TBL_SLACK more entries).
+        static_assert(HVM_TBL_SLACK >= 8 && HVM_PRE_NODE % 4 == 0, "table reads stay inside the tables");
 #pragma unroll
         for (int c = 0; c < 8; c += 4) {
           u32 vs[4];
