@@ -789,7 +789,12 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     bool oper1 = false;
     u32 cv = 0;
     if (run && rule == HVM_OPER) {
-      B2 = enter<LOC>(A, N, B2);
+      // The reference first follows B2's var chain (src/hvm.c:1024).  Here B2
+      // is used as it is: OPER-1 links it, and the linker follows the chain
+      // itself; OPER-2 stores it in the new node's second port, which no rule
+      // inspects (OPER-1 vs OPER-2 reads the first port), and the chain is
+      // followed when that port is linked.  The result and ITRS are the same;
+      // the step saves a dependent exchange.
       oper1 = hvm_get_tag(B1) == HVM_NUM;
       if (oper1) {
         cv = hvm_operate(hvm_get_val(a), hvm_get_val(B1));
