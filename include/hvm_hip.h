@@ -47,7 +47,7 @@ typedef struct hvm_hip_cfg {
   uint32_t flags;       /* HVM_HIP_NO_READBACK: skip the result text          */
   uint32_t time_limit_s;/* abandon evaluation after this many seconds (0: none) */
   uint32_t idle_pct;    /* end a turn early once this % of waves are idle
-                           (0: 50; above 100: never end early)                */
+                           (0: 25; above 100: never end early)                */
 } hvm_hip_cfg;
 
 #define HVM_HIP_NO_READBACK 1u
@@ -150,7 +150,8 @@ int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_net
  * {24, 96, 512, 4096} steps, i.e. how long waves work on their own redexes
  * before the HBM bag redistributes them.  It never changes what an
  * interaction does, so normal form and ITRS do not depend on it.
- *   HVM_HIP_SCHED_PLAIN   fixed heuristic on the number of pending redexes
+ *   HVM_HIP_SCHED_PLAIN   the longest turn (4096 steps), cut short by the
+ *                         idle-quit rule (hvm_hip_cfg.idle_pct)
  *   HVM_HIP_SCHED_LEARNED argmax of a 32->64->64->4 MLP (fp16 params,
  *                         HVM_HIP_MLP_PARAMS values) evaluated on the device;
  *                         it replaces the heuristic's choice only when its

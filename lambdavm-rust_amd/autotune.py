@@ -6,8 +6,8 @@
 The knobs are the ones a caller can set per context (include/hvm_hip.h,
 hvm_hip_cfg):
   * waves        — evaluator workgroups (one 64-lane wave each) per turn, as
-                   a multiple of the CU count (the default fills every CU as
-                   far as registers and LDS allow);
+                   a fraction of the default, which fills every CU as far as
+                   registers and LDS allow;
   * idle_pct     — a turn ends once this % of the waves ran out of work, so
                    that the output bag redistributes the busy waves' redexes;
   * turn_steps   — a fixed share interval (steps per turn) instead of the
@@ -83,11 +83,10 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
     base = dict(base_cfg or {})
     t_def, ident, r0 = measure(w, base, repeats)
     waves_def = r0.diag.get("waves", 0)
-    per_cu = 7
-    cus = max(1, waves_def // per_cu)
     log(f"[{w.name}] default: {t_def:.2f} ms, waves {waves_def}, itrs {ident[0]}")
     space = space or [
-        ("waves", [cus * k for k in (2, 3, 4, 5, 6)] + [0]),
+        # fractions of the default (all resident waves)
+        ("waves", [waves_def * k // 8 for k in (2, 4, 6, 7)] + [0]),
         ("idle_pct", [10, 25, 75, 101, 0]),
         ("turn_steps", [96, 512, 4096, 0]),
         ("flags", [H.LOCAL_NET, 0]),

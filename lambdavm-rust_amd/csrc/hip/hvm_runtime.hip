@@ -344,8 +344,8 @@ struct hvm_hip_ctx {
     if (cfg.flags & HVM_HIP_PROFILE) a.flags |= HVM_TURN_PROFILE;
     if (cfg.flags & HVM_HIP_LOCAL_NET) a.flags |= HVM_TURN_LOCAL;
     a.lspill = lspill;
-    // a turn ends early once idle_pct% of the waves ran out of work (default 50%)
-    u32 pct = cfg.idle_pct ? (cfg.idle_pct > 100 ? 0u : cfg.idle_pct) : 50u;
+    // a turn ends early once idle_pct% of the waves ran out of work (default 25%)
+    u32 pct = cfg.idle_pct ? (cfg.idle_pct > 100 ? 0u : cfg.idle_pct) : 25u;
     a.idle_quit = pct ? std::max<u32>(1u, (u32)(((u64)nwaves * pct) / 100)) : 0u;
     return a;
   }
