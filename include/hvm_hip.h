@@ -146,12 +146,14 @@ int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_net
 /* between fixed SEED/GROW/WORK modes, src/hvm.cu:1509-1548)                */
 /* ------------------------------------------------------------------------ */
 
-/* After every turn the scheduler picks the next turn's length from
- * {24, 96, 512, 4096} steps, i.e. how long waves work on their own redexes
- * before the HBM bag redistributes them.  It never changes what an
- * interaction does, so normal form and ITRS do not depend on it.
- *   HVM_HIP_SCHED_PLAIN   the longest turn (4096 steps), cut short by the
- *                         idle-quit rule (hvm_hip_cfg.idle_pct)
+/* After every turn the scheduler picks how the next turn shares work: one of
+ * 4 actions (turn length in steps, idle-quit share of the waves):
+ * (4096, 10%), (4096, idle_pct), (4096, 50%), (512, idle_pct) — how long
+ * waves work on their own redexes, and how many of them must run dry, before
+ * the HBM bag redistributes redexes.  It never changes what an interaction
+ * does, so normal form and ITRS do not depend on it.
+ *   HVM_HIP_SCHED_PLAIN   action 1: the longest turn, cut short once
+ *                         hvm_hip_cfg.idle_pct of the waves are idle
  *   HVM_HIP_SCHED_LEARNED argmax of a 32->64->64->4 MLP (fp16 params,
  *                         HVM_HIP_MLP_PARAMS values) evaluated on the device;
  *                         it replaces the heuristic's choice only when its

@@ -112,7 +112,8 @@ struct HvmCtl {
   uint32_t done;        // set once a turn leaves no redex (or on error)
   uint32_t fixed_steps; // if nonzero, every turn runs this many steps
   uint32_t idle_waves;  // waves of the current turn that ran out of work
-  uint32_t pad[6];
+  uint32_t idle_quit;   // the current turn ends once this many waves are idle (0: never early)
+  uint32_t pad[5];
 };
 
 // ---------------------------------------------------------------------------
@@ -178,7 +179,7 @@ struct HvmTurnArgs {
   uint32_t root_vars;              // var slots [0, root_vars) are roots, never allocated
   HvmSched* sched;
   uint32_t flags;                  // HVM_TURN_* behaviour switches
-  uint32_t idle_quit;              // a turn ends once this many waves are idle (0: never early)
+  uint32_t idle_quit;              // the plain scheduler's idle-quit wave count (0: never early)
   unsigned long long* lspill;      // local-net evaluator: HVM_LSPILL entries per wave
 };
 

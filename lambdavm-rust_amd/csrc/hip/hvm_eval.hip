@@ -499,6 +499,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   if (C->done) return;
   const u32 n_in = C->n_in;
   const u32 max_steps = C->max_steps;
+  const u32 idle_quit = C->idle_quit;
   const u64* bag_in = A.bag[C->parity];
   u64* bag_out = A.bag[C->parity ^ 1u];
 
@@ -625,7 +626,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     if ((step & 15) == 15) {
       if (!evac) {
         u32 idle = __hip_atomic_load(&A.ctl->idle_waves, __ATOMIC_RELAXED, AGENT);
-        if (A.idle_quit && idle >= A.idle_quit) {
+        if (idle_quit && idle >= idle_quit) {
           if (!LOC) break;
           evac = true;
         }
@@ -1158,18 +1159,21 @@ __global__ __launch_bounds__(256) void hvm_gather_kernel(HvmGatherArgs G) {
 // all 64 lanes evaluate the MLP in learned mode.
 // ---------------------------------------------------------------------------
 
-__device__ const u32 kTurnLen[HVM_ACTIONS] = {24, 96, 512, 4096};
+// Scheduler actions: turn length and idle-quit share (0: the configured one,
+// hvm_hip_cfg.idle_pct).  Action 1 is the plain heuristic's.
+__device__ const u32 kTurnLen[HVM_ACTIONS] = {4096, 4096, 4096, 512};
+__device__ const u32 kIdlePct[HVM_ACTIONS] = {10, 0, 50, 0};
 
 // The fixed heuristic: short turns while spreading a few redexes over the
 // waves, medium while growing, long once every lane can have work.
-// The plain heuristic: the longest turn.  A turn also ends once the idle-quit
-// share of the waves has run out of work, which shortens turns by itself when
-// redexes are scarce (measured: shorter fixed turns were slower on every
-// workload here, by up to 19%).
+// The plain heuristic: the longest turn with the configured idle-quit share.
+// A turn ends once that share of the waves has run out of work, which
+// shortens turns by itself when redexes are scarce (measured: shorter fixed
+// turns were slower on every workload here, by up to 19%).
 __host__ __device__ __forceinline__ u32 plain_action(u32 n, u32 nwaves) {
   (void)n;
   (void)nwaves;
-  return HVM_ACTIONS - 1;
+  return 1;
 }
 
 __device__ __forceinline__ float log2p(float x) { return __log2f(1.0f + x); }
@@ -1284,6 +1288,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
   }
   if (t == 0) {
     c->max_steps = c->fixed_steps ? c->fixed_steps : kTurnLen[action];
+    c->idle_quit = kIdlePct[action] ? max(1u, (A.nwaves * kIdlePct[action]) / 100u) : A.idle_quit;
     s->last_action = action;
     if (s->trace && s->trace_len < s->trace_cap) {
       float* rec = s->trace + (size_t)s->trace_len * HVM_TRACE_REC;
@@ -1317,7 +1322,7 @@ extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, uint32_t clock_kh
 // Turn length of the first turn (the host boots it; later turns are chosen
 // on the device).
 extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves) {
-  static const uint32_t len[HVM_ACTIONS] = {24, 96, 512, 4096};
+  static const uint32_t len[HVM_ACTIONS] = {4096, 4096, 4096, 512};
   return len[plain_action(n, nwaves)];
 }
 

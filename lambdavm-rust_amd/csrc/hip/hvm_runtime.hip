@@ -314,6 +314,7 @@ struct hvm_hip_ctx {
     c.in_head = 1;
     c.parity = 0;
     c.fixed_steps = cfg.turn_steps;
+    c.idle_quit = turn_args().idle_quit;
     c.max_steps = cfg.turn_steps ? cfg.turn_steps : hvm_turn_policy_host(1, nwaves);
     c.turns = 0;
     c.done = 0;
@@ -611,6 +612,7 @@ struct hvm_hip_ctx {
     c.n_in = n;
     c.in_head = std::min<u32>(n, nwaves * HVM_WAVE);
     c.fixed_steps = cfg.turn_steps;
+    c.idle_quit = turn_args().idle_quit;
     c.max_steps = cfg.turn_steps ? cfg.turn_steps : hvm_turn_policy_host(n, nwaves);
     host_ctl = c;
     if (!ck(hipMemcpyAsync(ctl, &host_ctl, sizeof(HvmCtl), hipMemcpyHostToDevice, stream), "boot ctl")) return false;

@@ -31,7 +31,8 @@ LOCAL_NET = 8
 
 # Scheduler (include/hvm_hip.h)
 SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
-TURN_LENGTHS = (24, 96, 512, 4096)
+TURN_LENGTHS = (4096, 4096, 4096, 512)
+IDLE_PCTS = (10, None, 50, None)   # None: the configured idle_pct
 MLP_FEAT, MLP_HID, MLP_ACT = 32, 64, 4
 MLP_PARAMS = MLP_FEAT * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_ACT + MLP_ACT
 TRACE_REC = MLP_FEAT + 4
