@@ -60,6 +60,17 @@ struct HvmDevDef {
   uint32_t call_bytes; // 8N + 4V + 8(R+1), the canonical CALL byte count
 };
 
+// The def record the evaluator loads for a CALL: the header and the first
+// template pairs in one 128-byte record, so that one round trip brings them.
+// Further pairs are read from the flat pair array.
+#define HVM_PRE_NODE 8
+#define HVM_PRE_RBAG 4
+struct HvmDevDefX {
+  HvmDevDef h;
+  unsigned long long node[HVM_PRE_NODE];  // node_buf[0, 8), zero past nlen
+  unsigned long long rbag[HVM_PRE_RBAG];  // rbag_buf[0, 4), zero past rlen
+};
+
 // Statistic slots accumulated by the evaluator (same order as the oracle).
 enum {
   HVM_K_CALL = 0, HVM_K_CALL_COPY, HVM_K_VOID, HVM_K_ERAS, HVM_K_ANNI, HVM_K_COMM,
@@ -165,7 +176,7 @@ struct HvmTurnArgs {
   uint32_t node_slice, vars_slice; // slots per wave
   uint32_t node_pool_lo, vars_pool_lo; // shared overflow pools: [pool_lo, cap)
   // book
-  const HvmDevDef* defs;
+  const HvmDevDefX* defs;
   const unsigned long long* pairs;
   uint32_t ndefs;
   // bags: bag[ctl->parity] is the input, the other one the output

@@ -410,9 +410,11 @@ __device__ __forceinline__ bool scan_advance(Scan& s, u32 pool_lo, u32 pool_len,
 // Allocates need_n node slots and need_v var slots for the whole wave.  Each
 // round issues one coalesced probe load per buffer that still needs slots, so
 // both buffers are scanned concurrently.  All arguments are wave-uniform.
+// With `pre`, pre_n/pre_v are this lane's values of the first window of each
+// buffer, loaded earlier at the same cursor.
 __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u32 n_lo, u32 v_lo,
                                             HvmWaveState& cur, u32 need_n, u32 need_v, u64& probes, u32& oom,
-                                            u32* tbl_n, u32* tbl_v) {
+                                            u32* tbl_n, u32* tbl_v, bool pre, u64 pre_n, u32 pre_v) {
   const u32 lane = lane_id();
   Scan sn = {n_lo, A.node_slice, cur.node_cur, 0, need_n, 0, false};
   Scan sv = {v_lo, A.vars_slice, cur.vars_cur, 0, need_v, 0, false};
@@ -423,8 +425,9 @@ __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u3
     u32 rn, rv;
     u32 in_ = scan_idx(sn, lane, rn);
     u32 iv = scan_idx(sv, lane, rv);
-    u64 vn = pn ? __hip_atomic_load(&A.node[in_], __ATOMIC_RELAXED, AGENT) : 1ull;
-    u32 vv = pv ? __hip_atomic_load(&A.vars[iv], __ATOMIC_RELAXED, AGENT) : 1u;
+    u64 vn = !pn ? 1ull : pre ? pre_n : __hip_atomic_load(&A.node[in_], __ATOMIC_RELAXED, AGENT);
+    u32 vv = !pv ? 1u : pre ? pre_v : __hip_atomic_load(&A.vars[iv], __ATOMIC_RELAXED, AGENT);
+    pre = false;
     if (pn) {
       scan_window(A.node, sn, vn, in_, rn, 1u, ~0ull, tbl_n, &L.cursor_n);
       probes++;
@@ -477,12 +480,9 @@ __device__ __forceinline__ void put_link(u32* pa, u32* pb, u32& np, u32 p1, u32 
   np++;
 }
 
-// Template pairs of a CALL loaded ahead of use (the rest as they are needed).
-#ifndef HVM_PRE_NODE
-#define HVM_PRE_NODE 8
-#endif
-#ifndef HVM_PRE_RBAG
-#define HVM_PRE_RBAG 4
+
+#ifndef HVM_EARLY_TAKES
+#define HVM_EARLY_TAKES 1
 #endif
 
 // Steps a wave may spend draining its local net before giving up.
@@ -654,40 +654,102 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       if (evac && (rule == HVM_CALL || rule == HVM_COMM || rule == HVM_OPER || rule == HVM_SWIT))
         rule = (is_loc_port(a) || is_loc_port(b)) ? HVM_SAVE : HVM_EMIT;
     }
-    bool copy = false;
-    HvmDevDef d;
-    d.rlen = d.nlen = d.vlen = 0; d.rbag_off = d.node_off = 0; d.root = 0; d.safe = 0; d.call_bytes = 0;
-    u32 nn = 0, nv = 0;   // slots from the net's main allocator (local with LOC)
+    // HBM evaluator: takes are issued before the reservation, so that their
+    // round trip is the one of the CALLs' def records and the allocator's
+    // first window; a lane that ends up waiting puts its nodes back.  The
+    // local-net evaluator takes after the reservation (P3).
+    constexpr bool EARLY = !LOC && HVM_EARLY_TAKES;
+    const bool take_a0 = has && (rule == HVM_ANNI || rule == HVM_COMM);
+    const bool take_b0 = has && (rule == HVM_ERAS || rule == HVM_ANNI || rule == HVM_COMM ||
+                                 rule == HVM_OPER || rule == HVM_SWIT || (LOC && rule == HVM_LEAK));
+    // slots from the net's main allocator (local with LOC), known here for
+    // every rule but CALL
+    u32 nn = 0, nv = 0;
     u32 xn = 0, xv = 0;   // HBM slots of LEAK / SAVE
     if (has) {
-      if (rule == HVM_CALL) {
-        u32 fid = hvm_get_val(a) & 0xFFFFFFFu;
-        d = A.defs[fid < A.ndefs ? fid : 0];
-        if (d.safe && hvm_get_tag(b) == HVM_DUP) copy = true;
-        else { nn = d.nlen; nv = d.vlen; }
-      } else if (rule == HVM_COMM) { nn = 4; nv = 4; }
+      if (rule == HVM_COMM) { nn = 4; nv = 4; }
       else if (rule == HVM_OPER) { nn = 1; }
       else if (rule == HVM_SWIT) { nn = hvm_get_u24(hvm_get_val(a)) == 0 ? 1 : 2; }
       else if (LOC && rule == HVM_LEAK) { xn = 1; xv = 2; }
       else if (LOC && rule == HVM_SAVE) { xv = (u32)is_loc_port(a) + (u32)is_loc_port(b); }
     }
+    // HBM evaluator: lanes whose rule is not CALL reserve table space now,
+    // and those that fit take their nodes at once, so that the takes share
+    // one round trip with the CALLs' def records and the allocator's first
+    // window.  CALL lanes reserve after their def record arrives, behind the
+    // others.  A lane that waits has taken nothing, so a slot read as free in
+    // this step stays free (a taken-and-put-back node could be handed out).
+    // The node and var counts are scanned together (16 bits each; a wave's
+    // total is at most 64 * HVM_GPU_DEF_MAX < 2^16).
+    u32 off_n = 0, off_v = 0;
+    bool run = false;
+    u64 NA = 0, NB = 0;
+    if (EARLY) {
+      u32 tot;
+      const u32 o = wave_scan(rule != HVM_CALL ? (nn | (nv << 16)) : 0u, &tot);
+      off_n = o & 0xFFFFu; off_v = o >> 16;
+      run = has && rule != HVM_CALL && off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
+      NA = (run && take_a0) ? node_take(A, hvm_get_val(a)) : 0ull;
+      NB = (run && take_b0) ? node_take(A, hvm_get_val(b)) : 0ull;
+    }
+    // A CALL's def record with its first template pairs (one 128-byte load).
+    bool copy = false;
+    HvmDevDefX dx;
+    if (has && rule == HVM_CALL) {
+      u32 fid = hvm_get_val(a) & 0xFFFFFFFu;
+      dx = A.defs[fid < A.ndefs ? fid : 0];
+    } else {
+      dx.h.rlen = dx.h.nlen = dx.h.vlen = 0; dx.h.rbag_off = dx.h.node_off = 0;
+      dx.h.root = 0; dx.h.safe = 0; dx.h.call_bytes = 0;
+#pragma unroll
+      for (int i = 0; i < HVM_PRE_NODE; ++i) dx.node[i] = 0;
+#pragma unroll
+      for (int i = 0; i < HVM_PRE_RBAG; ++i) dx.rbag[i] = 0;
+    }
+    // The allocator's first probe window, loaded in the same round trip
+    // (HBM evaluator; used by wave_alloc2 if this step allocates).
+    u64 pre_n = 1;
+    u32 pre_v = 1;
+    if (EARLY) {
+      u32 rel;
+      Scan s0n = {n_lo, A.node_slice, ws.node_cur, 0, 0, 0, false};
+      Scan s0v = {v_lo, A.vars_slice, ws.vars_cur, 0, 0, 0, false};
+      pre_n = __hip_atomic_load(&A.node[scan_idx(s0n, lane, rel)], __ATOMIC_RELAXED, AGENT);
+      pre_v = __hip_atomic_load(&A.vars[scan_idx(s0v, lane, rel)], __ATOMIC_RELAXED, AGENT);
+    }
+    const HvmDevDef& d = dx.h;
+    if (has && rule == HVM_CALL) {
+      if (d.safe && hvm_get_tag(b) == HVM_DUP) copy = true;
+      else { nn = d.nlen; nv = d.vlen; }
+    }
 
     // ---- P1: reserve allocation-table space (lanes that do not fit wait) ----
-    // node and var counts are scanned together (16 bits each; a wave's total
-    // is at most 64 * HVM_GPU_DEF_MAX < 2^16)
-    u32 tot_m, tot_x = 0;
-    u32 off_m = wave_scan(nn | (nv << 16), &tot_m);
-    u32 off_n = off_m & 0xFFFFu, off_v = off_m >> 16;
+    u32 tot_x = 0;
     u32 off_xn = 0, off_xv = 0;
-    u32 cap_n = HVM_TBL_NODE, cap_v = HVM_TBL_VARS;
-    if (LOC) {
-      u32 off_x = wave_scan(xn | (xv << 16), &tot_x);
-      off_xn = off_x & 0xFFFFu; off_xv = off_x >> 16;
-      cap_n = min(cap_n, __hip_atomic_load(&N->ntop, __ATOMIC_RELAXED, WGS));
-      cap_v = min(cap_v, __hip_atomic_load(&N->vtop, __ATOMIC_RELAXED, WGS));
+    if (EARLY) {
+      // CALL lanes, after the space the other lanes took
+      const u32 base_n = wave_max(run ? off_n + nn : 0), base_v = wave_max(run ? off_v + nv : 0);
+      u32 tot;
+      const u32 o = wave_scan(rule == HVM_CALL ? (nn | (nv << 16)) : 0u, &tot);
+      if (has && rule == HVM_CALL) {
+        off_n = base_n + (o & 0xFFFFu);
+        off_v = base_v + (o >> 16);
+        run = off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
+      }
+    } else {
+      u32 tot_m;
+      u32 off_m = wave_scan(nn | (nv << 16), &tot_m);
+      off_n = off_m & 0xFFFFu; off_v = off_m >> 16;
+      u32 cap_n = HVM_TBL_NODE, cap_v = HVM_TBL_VARS;
+      if (LOC) {
+        u32 off_x = wave_scan(xn | (xv << 16), &tot_x);
+        off_xn = off_x & 0xFFFFu; off_xv = off_x >> 16;
+        cap_n = min(cap_n, __hip_atomic_load(&N->ntop, __ATOMIC_RELAXED, WGS));
+        cap_v = min(cap_v, __hip_atomic_load(&N->vtop, __ATOMIC_RELAXED, WGS));
+      }
+      run = has && off_n + nn <= cap_n && off_v + nv <= cap_v &&
+            off_xn + xn <= HVM_TBL_X && off_xv + xv <= HVM_TBL_X;
     }
-    bool run = has && off_n + nn <= cap_n && off_v + nv <= cap_v &&
-               off_xn + xn <= HVM_TBL_X && off_xv + xv <= HVM_TBL_X;
     if (has && !run) defers++;
     if (LOC && !evac && !__ballot(run)) {
       // the local net is too full for any redex here: drain it
@@ -704,26 +766,18 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     if (run) ln.top--;
 
     HVM_PHASE(1);
-    // ---- P3: takes, issued before the allocator's probes so that both
-    //      round trips overlap ----
-    bool take_a = run && (rule == HVM_ANNI || rule == HVM_COMM);
-    bool take_b = run && (rule == HVM_ERAS || rule == HVM_ANNI || rule == HVM_COMM ||
-                          rule == HVM_OPER || rule == HVM_SWIT || (rule == HVM_CALL && copy) ||
-                          (LOC && rule == HVM_LEAK));
-    u64 NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
-    u64 NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
-    // A CALL's template: its first node and redex pairs are loaded now, so
-    // that their round trip overlaps the takes and the allocation.
-    u64 tpre[HVM_PRE_NODE], rpre[HVM_PRE_RBAG];
-    {
-      const bool inst = run && rule == HVM_CALL && !copy;
-#pragma unroll
-      for (int i = 0; i < HVM_PRE_NODE; ++i)
-        tpre[i] = (inst && (u32)i < d.nlen) ? A.pairs[d.node_off + i] : 0ull;
-#pragma unroll
-      for (int i = 0; i < HVM_PRE_RBAG; ++i)
-        rpre[i] = (inst && (u32)i < d.rlen) ? A.pairs[d.rbag_off + i] : 0ull;
+    // ---- P3: takes of the running lanes that have not taken yet (all of
+    //      them in the local-net evaluator; a safe def's copy rule) ----
+    const bool take_a = run && take_a0;
+    const bool take_b = run && (take_b0 || (rule == HVM_CALL && copy));
+    if (!EARLY) {
+      NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
+      NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
+    } else if (run && rule == HVM_CALL && copy) {
+      NB = node_take(A, hvm_get_val(b));
     }
+    const u64* tpre = dx.node;
+    const u64* rpre = dx.rbag;
 
     // ---- P2: allocation (wave-cooperative, nodes and vars together) ----
     if (LOC) {
@@ -743,7 +797,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       __syncthreads();
       if (need_xn || need_xv) {
         u32 oom = 0;
-        if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_xn, need_xv, probes, oom, N->xtn, N->xtv)) {
+        if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_xn, need_xv, probes, oom, N->xtn, N->xtv, false, 0, 0)) {
           atomicOr(&A.ctl->err, oom);
           if (run) ln.top++;
           break;
@@ -751,7 +805,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       }
     } else if (need_n || need_v) {
       u32 oom = 0;
-      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom, L.tbl_n, L.tbl_v)) {
+      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom, L.tbl_n, L.tbl_v, true, pre_n, pre_v)) {
         // out of slots in the slice and the pool: the evaluation stops with an error
         atomicOr(&A.ctl->err, oom);
         if (run) ln.top++;

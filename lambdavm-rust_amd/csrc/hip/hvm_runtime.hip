@@ -68,7 +68,7 @@ struct hvm_hip_ctx {
   uint32_t clock_khz = 100000;
   // book
   hvm::Book book;
-  HvmDevDef* d_defs = nullptr;
+  HvmDevDefX* d_defs = nullptr;
   u64* d_pairs = nullptr;
   u32 ndefs = 0;
   bool loaded = false;
@@ -204,11 +204,20 @@ struct hvm_hip_ctx {
       x.call_bytes = 8 * x.nlen + 4 * x.vlen + 8 * (x.rlen + 1);
     }
     if (pairs.empty()) pairs.push_back(0);
+    // records with the first template pairs inline (HvmDevDefX)
+    std::vector<HvmDevDefX> defx(defs.size());
+    for (size_t f = 0; f < defs.size(); ++f) {
+      HvmDevDefX& x = defx[f];
+      memset(&x, 0, sizeof x);
+      x.h = defs[f];
+      for (u32 i = 0; i < HVM_PRE_NODE && i < x.h.nlen; ++i) x.node[i] = pairs[x.h.node_off + i];
+      for (u32 i = 0; i < HVM_PRE_RBAG && i < x.h.rlen; ++i) x.rbag[i] = pairs[x.h.rbag_off + i];
+    }
     if (d_defs) { hipFree(d_defs); d_defs = nullptr; }
     if (d_pairs) { hipFree(d_pairs); d_pairs = nullptr; }
-    if (!ck(hipMalloc(&d_defs, defs.size() * sizeof(HvmDevDef)), "hipMalloc(defs)")) return false;
+    if (!ck(hipMalloc(&d_defs, defx.size() * sizeof(HvmDevDefX)), "hipMalloc(defs)")) return false;
     if (!ck(hipMalloc(&d_pairs, pairs.size() * 8), "hipMalloc(pairs)")) return false;
-    if (!ck(hipMemcpy(d_defs, defs.data(), defs.size() * sizeof(HvmDevDef), hipMemcpyHostToDevice), "copy defs")) return false;
+    if (!ck(hipMemcpy(d_defs, defx.data(), defx.size() * sizeof(HvmDevDefX), hipMemcpyHostToDevice), "copy defs")) return false;
     if (!ck(hipMemcpy(d_pairs, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice), "copy pairs")) return false;
     ndefs = (u32)defs.size();
     book = std::move(b);
@@ -390,7 +399,12 @@ struct hvm_hip_ctx {
       launches += chunk;
       turns += ran;
       turns_before = host_ctl.turns;
-      if (host_ctl.err) return fail("evaluator error flags 0x" + std::to_string(host_ctl.err));
+      if (host_ctl.err) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "evaluator error flags 0x%x (turn %u, output bag %u of %u)", host_ctl.err,
+                 host_ctl.turns, host_ctl.n_out, bag_cap);
+        return fail(msg);
+      }
       if (host_ctl.done) break;
       if (chunk < max_chunk) chunk *= 2;
     }
