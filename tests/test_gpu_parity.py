@@ -116,3 +116,43 @@ def test_oversized_def_is_rejected(hip, ctx):
         tree = f"(* {tree})"
     with pytest.raises(hip.HvmError):
         ctx.load(hip.compile_book(f"@main = {tree}"))
+
+
+# Failure handling, forced from the host: a net or bag far too small for the
+# program.  The evaluator must stop with the matching error flag (every kernel
+# exits normally) and report it through the status code, never a result.
+OOM_NODE, OOM_VARS, BAG_OVERFLOW = 1, 2, 4
+
+
+def error_flags(e) -> int:
+    """The evaluator's error flags named in an HvmError message."""
+    import re
+    m = re.search(r"(?:flags|err=)\s*0x([0-9a-f]+)", str(e.value))
+    assert m, str(e.value)
+    return int(m.group(1), 16)
+
+
+def test_out_of_memory_is_reported(hip):
+    code, _ = P.bitonic(12, 0)   # peak live nodes far above the 2^15-slot net
+    with pytest.raises(hip.HvmError) as e:
+        hip.run_code(code, time_limit_s=60, waves=64, node_cap=1 << 15, vars_cap=1 << 15)
+    assert error_flags(e) & (OOM_NODE | OOM_VARS)
+
+
+def test_bag_overflow_is_reported(hip):
+    code, _ = P.sum_tree(16)
+    with pytest.raises(hip.HvmError) as e:
+        hip.run_code(code, time_limit_s=60, bag_cap=64)
+    assert error_flags(e) & BAG_OVERFLOW
+
+
+def test_context_recovers_after_an_error(golden, hip):
+    # an error leaves the context usable: the next evaluation starts clean
+    case = next(c for c in golden["workloads"] if c["generator"] == "sum_tree" and c["params"] == {"depth": 4})
+    with hip.Context(time_limit_s=60, bag_cap=64) as ctx:
+        ctx.load(hip.compile_book(P.sum_tree(16)[0]))
+        with pytest.raises(hip.HvmError):
+            ctx.eval()
+        ctx.load(hip.compile_book(P.sum_tree(4)[0]))
+        r = ctx.eval()
+        assert r.err == 0 and r.itrs == case["itrs"] and r.result == "16"
