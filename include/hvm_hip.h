@@ -26,9 +26,10 @@ extern "C" {
 
 /* Replaces `hvm_cu` (src/hvm.cu:2320-2397) and `hvm_c` (src/hvm.c:1743-1791).
  * Loads the book (src/hvm.cu:1898-1947 semantics; prints "failed to load
- * book" to stderr and returns on failure), evaluates @main on the GPU with
- * the run-cu IO driver's readback (src/run.cu:857-925) and prints, exactly
- * as run-cu does:
+ * book" to stderr and returns on failure), evaluates @main on the GPU under
+ * the IO driver (src/run.c:740-809: READ/OPEN/CLOSE/FLUSH/WRITE/SEEK/
+ * GET_TIME/SLEEP run on the host; DL_OPEN/DL_CALL/DL_CLOSE answer with an IO
+ * error) and prints, exactly as run-cu does:
  *     Result: <tree>\n- ITRS: <n>\n- LEAK: <n>\n- TIME: <s>s\n- MIPS: <m>\n
  * TIME is wall clock from boot to the result being read back.  A NULL
  * buffer is rejected (the reference would run an uninitialised book). */
@@ -102,7 +103,8 @@ typedef struct hvm_hip_out {
 /* Status codes */
 #define HVM_HIP_OK          0
 #define HVM_HIP_E_BOOK      1   /* book failed to load / exceeds limits       */
-#define HVM_HIP_E_IO        2   /* program requested IO (not supported here)  */
+#define HVM_HIP_E_IO        2   /* an IO call's result did not fit the net's host
+                                   region, or a batched net asked for IO      */
 #define HVM_HIP_E_DEVICE    3   /* HIP error or no GPU                        */
 #define HVM_HIP_E_EVAL      4   /* evaluator error (see out->err)             */
 #define HVM_HIP_E_ARG       5   /* bad argument                               */

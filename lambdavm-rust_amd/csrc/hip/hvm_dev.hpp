@@ -174,7 +174,8 @@ struct HvmTurnArgs {
   uint32_t* vars;
   uint32_t node_cap, vars_cap;     // slots (slot 0 never allocated)
   uint32_t node_slice, vars_slice; // slots per wave
-  uint32_t node_pool_lo, vars_pool_lo; // shared overflow pools: [pool_lo, cap)
+  uint32_t node_pool_lo, vars_pool_lo; // shared overflow pools: [pool_lo, pool_hi)
+  uint32_t node_pool_hi, vars_pool_hi; // slots from pool_hi to cap belong to the host (IO)
   // book
   const HvmDevDefX* defs;
   const unsigned long long* pairs;

@@ -431,12 +431,12 @@ __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u3
     if (pn) {
       scan_window(A.node, sn, vn, in_, rn, 1u, ~0ull, tbl_n, &L.cursor_n);
       probes++;
-      if (!scan_advance(sn, A.node_pool_lo, A.node_cap - A.node_pool_lo, cur.node_ovf)) { oom = HVM_ERR_OOM_NODE; return false; }
+      if (!scan_advance(sn, A.node_pool_lo, A.node_pool_hi - A.node_pool_lo, cur.node_ovf)) { oom = HVM_ERR_OOM_NODE; return false; }
     }
     if (pv) {
       scan_window(A.vars, sv, vv, iv, rv, A.root_vars, (u32)HVM_NONE, tbl_v, &L.cursor_v);
       probes++;
-      if (!scan_advance(sv, A.vars_pool_lo, A.vars_cap - A.vars_pool_lo, cur.vars_ovf)) { oom = HVM_ERR_OOM_VARS; return false; }
+      if (!scan_advance(sv, A.vars_pool_lo, A.vars_pool_hi - A.vars_pool_lo, cur.vars_ovf)) { oom = HVM_ERR_OOM_VARS; return false; }
     }
   }
   __syncthreads();
@@ -527,10 +527,10 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   if (ws.node_cur < n_lo || ws.node_cur >= n_lo + A.node_slice) ws.node_cur = n_lo;
   if (ws.vars_cur < v_lo || ws.vars_cur >= v_lo + A.vars_slice) ws.vars_cur = v_lo;
   // overflow-pool cursors start spread out, one region per wave
-  if (ws.node_ovf < A.node_pool_lo || ws.node_ovf >= A.node_cap)
-    ws.node_ovf = A.node_pool_lo + (u32)(((u64)(A.node_cap - A.node_pool_lo) * w) / A.nwaves);
-  if (ws.vars_ovf < A.vars_pool_lo || ws.vars_ovf >= A.vars_cap)
-    ws.vars_ovf = A.vars_pool_lo + (u32)(((u64)(A.vars_cap - A.vars_pool_lo) * w) / A.nwaves);
+  if (ws.node_ovf < A.node_pool_lo || ws.node_ovf >= A.node_pool_hi)
+    ws.node_ovf = A.node_pool_lo + (u32)(((u64)(A.node_pool_hi - A.node_pool_lo) * w) / A.nwaves);
+  if (ws.vars_ovf < A.vars_pool_lo || ws.vars_ovf >= A.vars_pool_hi)
+    ws.vars_ovf = A.vars_pool_lo + (u32)(((u64)(A.vars_pool_hi - A.vars_pool_lo) * w) / A.nwaves);
 
   // Initial spread: lane l of wave w takes item w + l*W, so a small bag is
   // spread one redex per wave before any wave takes a second one.

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cerrno>
 #include <cstdio>
+#include <ctime>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -52,7 +54,7 @@ struct hvm_hip_ctx {
   u32* vars = nullptr;
   u32 node_cap = 0, vars_cap = 0;
   u32 nwaves = 0, node_slice = 0, vars_slice = 0;
-  u32 node_pool_lo = 0, vars_pool_lo = 0;
+  u32 node_pool_lo = 0, vars_pool_lo = 0, node_pool_hi = 0, vars_pool_hi = 0;
   // bags + control
   u64* bag[2] = {nullptr, nullptr};
   u32 bag_cap = 0;
@@ -122,6 +124,10 @@ struct hvm_hip_ctx {
     vars_slice = (u32)(((u64)(vars_cap - root_vars) * 3 / 4) / nwaves);
     node_pool_lo = 1 + nwaves * node_slice;
     vars_pool_lo = root_vars + nwaves * vars_slice;
+    // the top of each buffer is the host's, for the nodes and vars the IO
+    // driver creates (up to 2^16 slots, half the overflow pool at most)
+    node_pool_hi = node_cap - std::min<u32>(1u << 16, (node_cap - node_pool_lo) / 2);
+    vars_pool_hi = vars_cap - std::min<u32>(1u << 16, (vars_cap - vars_pool_lo) / 2);
     if (node_slice < 4 * HVM_WAVE || vars_slice < 4 * HVM_WAVE) return fail("net too small for the wave count");
     if (!ck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
     if (!ck(hipEventCreate(&ev0), "hipEventCreate") || !ck(hipEventCreate(&ev1), "hipEventCreate")) return false;
@@ -341,6 +347,7 @@ struct hvm_hip_ctx {
     a.node_cap = node_cap; a.vars_cap = vars_cap;
     a.node_slice = node_slice; a.vars_slice = vars_slice;
     a.node_pool_lo = node_pool_lo; a.vars_pool_lo = vars_pool_lo;
+    a.node_pool_hi = node_pool_hi; a.vars_pool_hi = vars_pool_hi;
     a.defs = d_defs; a.pairs = d_pairs; a.ndefs = ndefs;
     a.bag[0] = bag[0]; a.bag[1] = bag[1];
     a.bag_cap = bag_cap;
@@ -452,17 +459,264 @@ struct hvm_hip_ctx {
     return true;
   }
 
-  // src/run.cu:857-925 without the IO calls: returns 2 when the program asks
-  // for an IO call (out of scope for this evaluator).
+  // -------------------------------------------------------------------------
+  // IO driver (src/run.c:1-809; the CUDA runtime's is src/run.cu:67-925).
+  // An IO program's normal form is a λ-encoded IO/Call(magic, name, argument,
+  // continuation): the host reads the call back, performs it, writes the
+  // result into the net as new nodes and normalises again, until the program
+  // returns IO/Done.  Every normalisation pass runs on the GPU; reading ports
+  // back and writing the results are host memory copies.
+  // -------------------------------------------------------------------------
+
+  // Host-created nodes and vars live in [pool_hi, cap) of each buffer, which
+  // the device never allocates; the device frees them (zeroes) when it takes
+  // them, like any other slot.  The region is read once per IO call.
+  std::vector<u64> hnode;
+  std::vector<u32> hvar;
+  u32 hnode_cur = 0, hvar_cur = 0;
+
+  bool host_region_read() {
+    hnode.resize(node_cap - node_pool_hi);
+    hvar.resize(vars_cap - vars_pool_hi);
+    hnode_cur = hvar_cur = 0;
+    return ck(hipMemcpy(hnode.data(), node + node_pool_hi, hnode.size() * 8, hipMemcpyDeviceToHost), "io region") &&
+           ck(hipMemcpy(hvar.data(), vars + vars_pool_hi, hvar.size() * 4, hipMemcpyDeviceToHost), "io region");
+  }
+  // A fresh var (NONE) in the host region; 0 when the region is full.
+  u32 host_var() {
+    for (; hvar_cur < hvar.size(); ++hvar_cur) {
+      if (hvar[hvar_cur] == 0) {
+        u32 v = vars_pool_hi + hvar_cur;
+        hvar[hvar_cur] = HVM_NONE;
+        vars_store(v, HVM_NONE);
+        return v;
+      }
+    }
+    return 0;
+  }
+  // A new node holding `val` in the host region; 0 when the region is full.
+  u32 host_node(u64 val) {
+    for (; hnode_cur < hnode.size(); ++hnode_cur) {
+      if (hnode[hnode_cur] == 0) {
+        u32 loc = node_pool_hi + hnode_cur;
+        hnode[hnode_cur] = val;
+        hipMemcpy(node + loc, &val, 8, hipMemcpyHostToDevice);
+        return loc;
+      }
+    }
+    return 0;
+  }
+  bool io_full = false;   // the host region ran out during this call
+  u32 nod(u32 tag, u64 val) {
+    u32 loc = host_node(val);
+    if (!loc) io_full = true;
+    return hvm_new_port(tag, loc);
+  }
+  static u32 num_u24(u32 v) { return hvm_new_port(HVM_NUM, hvm_new_u24(v)); }
+  static u32 num_i24(int32_t v) { return hvm_new_port(HVM_NUM, hvm_new_i24(v)); }
+
+  // λ-encoded constructors, src/run.c:184-375 (tags: list NIL 0 / CONS 1,
+  // result OK 0 / ERR 1, IO error TYPE 0 / NAME 1 / INNER 2).
+  u32 ctr_var() { u32 v = host_var(); if (!v) io_full = true; return hvm_new_port(HVM_VAR, v); }
+  u32 inject_nil() {                      // λt (t NIL)
+    u32 x = ctr_var();
+    u32 app = nod(HVM_CON, hvm_new_pair(num_u24(0), x));
+    return nod(HVM_CON, hvm_new_pair(app, x));
+  }
+  u32 inject_cons(u32 head, u32 tail) {   // λt (((t CONS) head) tail)
+    u32 x = ctr_var();
+    u32 a3 = nod(HVM_CON, hvm_new_pair(tail, x));
+    u32 a2 = nod(HVM_CON, hvm_new_pair(head, a3));
+    u32 a1 = nod(HVM_CON, hvm_new_pair(num_u24(1), a2));
+    return nod(HVM_CON, hvm_new_pair(a1, x));
+  }
+  u32 inject_bytes(const std::string& b) {
+    u32 p = inject_nil();
+    for (size_t i = b.size(); i-- > 0;) p = inject_cons(num_u24((unsigned char)b[i]), p);
+    return p;
+  }
+  u32 inject_tagged(u32 tag, u32 val) {   // λt ((t TAG) val)
+    u32 x = ctr_var();
+    u32 a2 = nod(HVM_CON, hvm_new_pair(val, x));
+    u32 a1 = nod(HVM_CON, hvm_new_pair(num_u24(tag), a2));
+    return nod(HVM_CON, hvm_new_pair(a1, x));
+  }
+  u32 inject_ok(u32 val) { return inject_tagged(0, val); }
+  u32 inject_err(u32 val) { return inject_tagged(1, val); }
+  u32 inject_io_err(u32 tag, u32 val) {
+    if (tag <= 1) {                       // IOError/Type, IOError/Name: λt (t TAG)
+      u32 x = ctr_var();
+      u32 app = nod(HVM_CON, hvm_new_pair(num_u24(tag), x));
+      return inject_err(nod(HVM_CON, hvm_new_pair(app, x)));
+    }
+    return inject_err(inject_tagged(2, val));   // IOError/Inner(val)
+  }
+  u32 io_err_type() { return inject_io_err(0, 0); }
+  u32 io_err_name() { return inject_io_err(1, 0); }
+  u32 io_err_inner(u32 val) { return inject_io_err(2, val); }
+  u32 io_err_errno(int e) { return io_err_inner(num_i24(e)); }
+  u32 io_err_str(const std::string& msg) { return io_err_inner(inject_bytes(msg)); }
+
+  // src/run.c:103-117: a right-nested CON tuple of at most `size` elements.
+  bool readback_tup(u32 port, u32 size, std::vector<u32>& elems) {
+    elems.clear();
+    for (u32 it = 0; it < (1u << 24) && hvm_get_tag(port) == HVM_CON && elems.size() + 1 < size; ++it) {
+      u64 nd = node_load(hvm_get_val(port));
+      u32 e;
+      if (!expand(hvm_get_fst(nd), e)) return false;
+      elems.push_back(e);
+      if (!expand(hvm_get_snd(nd), port)) return false;
+    }
+    elems.push_back(port);
+    return true;
+  }
+  // src/run.c:123-159: a λ-encoded list of bytes, one normalisation per
+  // element (the tail is booted against ROOT and read from there).
+  bool readback_bytes(u32 port, std::string& out) {
+    out.clear();
+    for (u64 it = 0; it < (1ull << 32); ++it) {
+      if (!normalize()) return false;
+      Ctr c;
+      if (!readback_ctr(peek(port), c)) return false;
+      if (c.tag != 1 || c.args_len != 2 || hvm_get_tag(c.args[0]) != HVM_NUM) break;
+      out.push_back((char)hvm_get_u24(hvm_get_val(c.args[0])));
+      if (!boot(hvm_new_pair(c.args[1], HVM_ROOT))) return false;
+      port = HVM_ROOT;
+    }
+    return true;
+  }
+  // src/run.c:169-180: the bytes as a C string (cut at the first NUL).
+  bool readback_str(u32 port, std::string& out) {
+    if (!readback_bytes(port, out)) return false;
+    out.resize(strlen(out.c_str()));
+    return true;
+  }
+
+  // Open files by descriptor; 0, 1 and 2 are stdin, stdout and stderr
+  // (src/run.c:380-411).
+  FILE* files[256] = {nullptr};
+  FILE* readback_file(u32 port) {
+    if (hvm_get_tag(port) != HVM_NUM) {
+      fprintf(stderr, "non-num where file descriptor was expected: %i\n", (int)hvm_get_tag(port));
+      return nullptr;
+    }
+    u32 fd = hvm_get_u24(hvm_get_val(port));
+    if (fd == 0) return stdin;
+    if (fd == 1) return stdout;
+    if (fd == 2) return stderr;
+    return fd < 256 ? files[fd] : nullptr;
+  }
+
+  // The IO functions of src/run.c:433-720, by name.  DL_OPEN/DL_CALL/DL_CLOSE
+  // would call into a dylib with the C runtime's in-memory net, which this
+  // evaluator does not have; they return an IO error.
+  bool io_call(const std::string& name, u32 argm, u32& ret) {
+    std::vector<u32> t;
+    if (name == "READ") {
+      if (!readback_tup(argm, 2, t)) return false;
+      if (t.size() != 2) { ret = io_err_type(); return true; }
+      FILE* fp = readback_file(t[0]);
+      u32 n = hvm_get_u24(hvm_get_val(t[1]));
+      if (!fp) { ret = io_err_errno(EBADF); return true; }
+      std::string buf(n, '\0');
+      size_t got = fread(&buf[0], 1, n, fp);
+      if (got != n && ferror(fp)) { ret = io_err_errno(ferror(fp)); return true; }
+      buf.resize(got);
+      ret = inject_ok(inject_bytes(buf));
+    } else if (name == "OPEN") {
+      if (!readback_tup(argm, 2, t)) return false;
+      if (t.size() != 2) { ret = io_err_type(); return true; }
+      std::string fname, mode;
+      if (!readback_str(t[0], fname) || !readback_str(t[1], mode)) return false;
+      for (u32 fd = 3; fd < 256; ++fd) {
+        if (!files[fd]) {
+          files[fd] = fopen(fname.c_str(), mode.c_str());
+          ret = files[fd] ? inject_ok(num_u24(fd)) : io_err_errno(errno);
+          return true;
+        }
+      }
+      ret = io_err_errno(EMFILE);
+    } else if (name == "CLOSE") {
+      FILE* fp = readback_file(argm);
+      if (!fp) { ret = io_err_errno(EBADF); return true; }
+      if (fclose(fp) != 0) { ret = io_err_errno(ferror(fp)); return true; }
+      u32 fd = hvm_get_u24(hvm_get_val(argm));
+      if (fd < 256) files[fd] = nullptr;
+      ret = inject_ok(hvm_new_port(HVM_ERA, 0));
+    } else if (name == "WRITE") {
+      if (!readback_tup(argm, 2, t)) return false;
+      if (t.size() != 2) { ret = io_err_type(); return true; }
+      FILE* fp = readback_file(t[0]);
+      std::string bytes;
+      if (!readback_bytes(t[1], bytes)) return false;
+      if (!fp) { ret = io_err_errno(EBADF); return true; }
+      if (fwrite(bytes.data(), 1, bytes.size(), fp) != bytes.size()) { ret = io_err_errno(ferror(fp)); return true; }
+      ret = inject_ok(hvm_new_port(HVM_ERA, 0));
+    } else if (name == "FLUSH") {
+      FILE* fp = readback_file(argm);
+      if (!fp) { ret = io_err_errno(EBADF); return true; }
+      if (fflush(fp) != 0) { ret = io_err_errno(ferror(fp)); return true; }
+      ret = inject_ok(hvm_new_port(HVM_ERA, 0));
+    } else if (name == "SEEK") {
+      if (!readback_tup(argm, 3, t)) return false;
+      if (t.size() != 3) { ret = io_err_type(); return true; }
+      FILE* fp = readback_file(t[0]);
+      int32_t off = hvm_get_i24(hvm_get_val(t[1]));
+      u32 whence = (u32)hvm_get_i24(hvm_get_val(t[2]));
+      if (!fp) { ret = io_err_errno(EBADF); return true; }
+      int w = whence == 0 ? SEEK_SET : whence == 1 ? SEEK_CUR : whence == 2 ? SEEK_END : -1;
+      if (w < 0) { ret = io_err_type(); return true; }
+      if (fseek(fp, off, w) != 0) { ret = io_err_errno(ferror(fp)); return true; }
+      ret = inject_ok(hvm_new_port(HVM_ERA, 0));
+    } else if (name == "GET_TIME") {
+      u64 ns = (u64)(now_s() * 1e9);
+      u32 hi = (u32)(ns >> 24) & 0xFFFFFFF, lo = (u32)(ns & 0xFFFFFFF);
+      ret = inject_ok(nod(HVM_CON, hvm_new_pair(num_u24(hi), num_u24(lo))));
+    } else if (name == "SLEEP") {
+      if (!readback_tup(argm, 2, t)) return false;
+      if (t.size() != 2) { ret = io_err_type(); return true; }
+      u64 ns = ((u64)hvm_get_u24(hvm_get_val(t[0])) << 24) | hvm_get_u24(hvm_get_val(t[1]));
+      struct timespec ts;
+      ts.tv_sec = (time_t)(ns / 1000000000ull);
+      ts.tv_nsec = (long)(ns % 1000000000ull);
+      nanosleep(&ts, nullptr);
+      ret = inject_ok(hvm_new_port(HVM_ERA, 0));
+    } else if (name == "DL_OPEN" || name == "DL_CALL" || name == "DL_CLOSE") {
+      ret = io_err_str("dynamic libraries are not supported by the HIP evaluator");
+    } else {
+      ret = io_err_name();
+    }
+    return true;
+  }
+
+  // src/run.c:740-809 (do_run_io).  Returns 0, or a status code on failure.
   int run_io() {
-    if (!normalize()) return HVM_HIP_E_EVAL;
-    Ctr c;
-    if (!readback_ctr(peek(HVM_ROOT), c)) return HVM_HIP_E_EVAL;
-    if (c.args_len < 1 || hvm_get_tag(c.args[0]) != HVM_CON) return 0;
-    u64 magic = node_load(hvm_get_val(c.args[0]));
-    if (hvm_get_val(hvm_get_fst(magic)) != hvm_new_u24(0xD0CA11) ||
-        hvm_get_val(hvm_get_snd(magic)) != hvm_new_u24(0xFF1FF1)) return 0;
-    if (c.tag == 1 && c.args_len == 4) return HVM_HIP_E_IO;
+    u32 port = HVM_ROOT;
+    for (u64 it = 0; it < (1ull << 40); ++it) {
+      if (!normalize()) return HVM_HIP_E_EVAL;
+      Ctr c;
+      if (!readback_ctr(peek(port), c)) return HVM_HIP_E_EVAL;
+      if (c.args_len < 1 || hvm_get_tag(c.args[0]) != HVM_CON) break;
+      u64 magic = node_load(hvm_get_val(c.args[0]));
+      if (hvm_get_val(hvm_get_fst(magic)) != hvm_new_u24(0xD0CA11) ||
+          hvm_get_val(hvm_get_snd(magic)) != hvm_new_u24(0xFF1FF1)) break;
+      if (c.tag != 1) break;                       // IO/Done (or not an IO call)
+      if (c.args_len != 4) {
+        fprintf(stderr, "invalid IO_CALL: args_len = %u\n", c.args_len);
+        break;
+      }
+      std::string name;
+      if (!readback_str(c.args[1], name)) return HVM_HIP_E_EVAL;
+      const u32 argm = c.args[2], cont = c.args[3];
+      io_full = false;
+      if (!host_region_read()) return HVM_HIP_E_DEVICE;
+      u32 ret = 0;
+      if (!io_call(name, argm, ret)) return HVM_HIP_E_EVAL;
+      u32 app = io_full ? 0 : host_node(hvm_new_pair(ret, HVM_ROOT));
+      if (io_full || !app) { fail("IO call results exceed the host region of the net"); return HVM_HIP_E_IO; }
+      if (!boot(hvm_new_pair(hvm_new_port(HVM_CON, app), cont))) return HVM_HIP_E_DEVICE;
+      port = HVM_ROOT;
+    }
     return 0;
   }
 
@@ -700,7 +954,6 @@ struct hvm_hip_ctx {
             continue;
           }
         }
-        t_norm += 0;
         if (cfg.flags & HVM_HIP_NO_READBACK) continue;
         std::string text;
         if (!print_result(peek(root), text)) return HVM_HIP_E_EVAL;
@@ -804,9 +1057,7 @@ extern "C" void hvm_hip(uint32_t* book_buffer) {
   setlinebuf(stderr);
   hvm_hip_out o;
   int rc = c->eval(&o);
-  if (rc == HVM_HIP_E_IO) {
-    fprintf(stderr, "IO calls are not supported by the HIP runtime\n");
-  } else if (rc != HVM_HIP_OK) {
+  if (rc != HVM_HIP_OK) {
     fprintf(stderr, "Failed to run the evaluator: %s\n", c->last_error.c_str());
     hvm_hip_destroy(c);
     exit(EXIT_FAILURE);

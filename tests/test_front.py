@@ -118,3 +118,24 @@ def test_parse_errors(hip):
     for bad in ["@main = (a", "main = *", "@main = $(a b", "@main = x & @nope ~ x"]:
         with pytest.raises(hip.HvmError):
             hip.compile_book(bad)
+
+
+def test_io_fixture_programs_compile(hip):
+    """tests/golden/io.json (the reference run-c's output on IO programs):
+    every program compiles, and the fixture records a result and ITRS for
+    each, so the GPU IO test compares against real reference output."""
+    import json
+    import os
+    with open(os.path.join(ROOT, "tests", "golden", "io.json")) as f:
+        fx = json.load(f)
+    names = [c["name"] for c in fx["programs"]]
+    assert "tests/programs/io/basic.hvm" in names and "tests/programs/io/echo_seek.hvm" in names
+    for c in fx["programs"]:
+        assert hip.compile_book(c["code"])
+        assert c["result"] and c["itrs"] > 0 and c["returncode"] == 0
+    # The program of this repository is the one the fixture was made from.
+    with open(os.path.join(ROOT, "tests", "programs", "io", "echo_seek.hvm")) as f:
+        assert f.read() == next(c for c in fx["programs"] if c["name"].endswith("echo_seek.hvm"))["code"]
+    # Its stdin READ and the 5 bytes read at offset 10 of the LICENSE fixture.
+    echo = next(c for c in fx["programs"] if c["name"].endswith("echo_seek.hvm"))
+    assert echo["lines"] == [fx["stdin"][:12] + fx["files"]["LICENSE"][10:15]]
