@@ -8,7 +8,10 @@ the evaluator to normal form (with the IO driver's readback expansions).
 Steps are timed with the wall clock around K steps; the dominant kernel
 (hvm_turn_kernel) is timed with HIP events on its own stream inside the
 library.  Parity (ITRS and normal form vs the reference run-c golden record)
-is checked on a warmup step outside the timed region.
+is checked on a warmup step outside the timed region.  The timed steps
+interpret the book as run-cu does; the compiled-CALL mode (def templates
+pre-reduced at load, same ITRS) is checked and timed after them and reported
+in "mode_compare" (`--compiled` swaps the two).
 
 Prints one JSON line (see the contract in the repository instructions).
 """
@@ -117,6 +120,10 @@ def main():
                     help="turn-length scheduler of the timed steps (the other one is timed after them)")
     ap.add_argument("--no-sched-compare", action="store_true",
                     help="do not time the other scheduler (profiling runs: only the bench's own evaluations)")
+    ap.add_argument("--compiled", action="store_true",
+                    help="time the evaluator with compiled CALLs (templates pre-reduced at load); by "
+                         "default the timed steps interpret the book as run-cu does and the compiled "
+                         "variant is timed after them")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (torch), to calibrate "
@@ -170,7 +177,8 @@ def main():
         else:
             ctx.set_scheduler(hvm_hip.SCHED_PLAIN)
 
-    ctx = hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK)
+    mode = hvm_hip.COMPILED_CALLS if args.compiled else 0
+    ctx = hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK | mode)
     ctx.load(words)
     use(ctx, args.sched)
     for _ in range(max(0, args.warmup - 1)):
@@ -192,6 +200,28 @@ def main():
         assert all(r.itrs == runs[0].itrs for r in o_runs), "ITRS changed with the scheduler"
         compare[other + "_mips"] = round(runs[0].itrs * args.steps / (t3 - t2) / 1e6, 2)
     ctx.close()
+
+    # the other evaluation mode (interpreted / compiled CALLs), after the timed
+    # region: parity against the golden record, then the same step count
+    variant = None
+    if not args.no_sched_compare:
+        vflag = 0 if args.compiled else hvm_hip.COMPILED_CALLS
+        with hvm_hip.Context(time_limit_s=600, flags=vflag) as vctx:
+            vctx.load(words)
+            rv = vctx.eval()
+        with hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK | vflag) as vctx:
+            vctx.load(words)
+            use(vctx, args.sched)
+            vctx.eval()
+            t4 = time.perf_counter()
+            v_runs = [vctx.eval() for _ in range(args.steps)]
+            t5 = time.perf_counter()
+        variant = {"mode": "interpreted" if args.compiled else "compiled_calls",
+                   "value": round(v_runs[0].itrs * args.steps / (t5 - t4) / 1e6, 2),
+                   "ms_per_step": round((t5 - t4) / args.steps * 1e3, 3),
+                   "itrs_match": gcase is not None and rv.itrs == gcase["itrs"],
+                   "nf_match": gcase is not None and
+                   hashlib.sha256(rv.canonical.encode()).hexdigest() == gcase["nf_sha256"]}
 
     itrs = runs[0].itrs
     assert all(r.itrs == itrs for r in runs), "ITRS changed between steps"
@@ -222,7 +252,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": f"bitonic sort of 2^{args.k} u24 keys (SplitMix64 seed 0), depth-{args.k} key tree",
-                   "keys": 1 << args.k, "itrs_per_step": itrs, "scheduler": args.sched},
+                   "keys": 1 << args.k, "itrs_per_step": itrs, "scheduler": args.sched,
+                   "mode": "compiled_calls" if args.compiled else "interpreted"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
@@ -233,6 +264,7 @@ def main():
         "eval_ms_per_step": round(sum(r.eval_ms for r in runs) / args.steps, 3),
         "parity": parity,
         "schedulers": compare,
+        "mode_compare": variant,
     }
     compare[args.sched + "_mips"] = out["value"]
     out["atomic_roofline"] = atomic_roofline(args.k, itrs * args.steps / (kernel_ms * 1e-3))

@@ -44,7 +44,7 @@ typedef struct hvm_hip_cfg {
   uint64_t vars_cap;    /* var slots in the net (0: 2^28)                     */
   uint32_t bag_cap;     /* redex-bag entries per bag (0: 2^24)                */
   uint32_t waves;       /* evaluator waves (0: fill every CU)                 */
-  uint32_t turn_steps;  /* steps per turn (0: adaptive)                       */
+  uint32_t turn_steps;  /* steps per turn (0: adaptive; at most 2^20)         */
   uint32_t flags;       /* HVM_HIP_NO_READBACK: skip the result text          */
   uint32_t time_limit_s;/* abandon evaluation after this many seconds (0: none) */
   uint32_t idle_pct;    /* end a turn early once this % of waves are idle
@@ -65,6 +65,15 @@ typedef struct hvm_hip_cfg {
  * moves what its local net still holds to HBM (src/hvm.cu:1426-1482).  Net
  * capacities are limited to 2^28 slots with it. */
 #define HVM_HIP_LOCAL_NET 8u
+/* Compiled CALLs (src/cmp.rs:10-351, the reference's compiled defs; its gen-cu
+ * leaves them out, src/main.rs:140-145): at load every def's template is
+ * pre-reduced — CALLs of REFs against the template's own nodes inlined, ANNIs
+ * between template nodes performed — within HVM_HIP_SPEC_NODES live nodes,
+ * and each expansion adds the def's static CALL and ANNI counts to ITRS and
+ * kind[], so ITRS, kind[] and the normal form equal run-c's.  Fewer nodes are
+ * created and taken per interaction counted. */
+#define HVM_HIP_COMPILED_CALLS 16u
+#define HVM_HIP_SPEC_NODES 64u
 
 /* Interaction kinds, in the order of hvm_hip_out.kind[]. */
 enum {
@@ -213,6 +222,18 @@ int hvm_front_compile(const char* code, uint32_t** words, size_t* n_words, char*
 /* Canonical form of a printed tree: parse, rename vars x0, x1, … in
  * first-occurrence order, re-print (tests/run.rs:112-146). */
 int hvm_front_canonical(const char* printed, char** out, char** err);
+
+/* The specialisation HVM_HIP_COMPILED_CALLS applies at load, on a wire buffer
+ * (no counterpart in the reference, whose compiled defs are C text,
+ * src/cmp.rs:10-34): returns 0, the specialised book as a malloc'd wire
+ * buffer, and a malloc'd array of 3 words per def: the static CALLs and ANNIs
+ * an expansion of the def adds to the counters, and the canonical bytes of one
+ * expansion (the unspecialised def's own plus the static interactions'),
+ * which replace the specialised template's own 8N+4V+8(R+1); all zero for a
+ * def left as it was.
+ * `node_budget` ≤ 256 bounds a def's live template nodes. */
+int hvm_front_specialize(const uint32_t* words, size_t n_words, uint32_t node_budget,
+                         uint32_t** out_words, size_t* out_n, uint32_t** counts, char** err);
 
 #ifdef __cplusplus
 }

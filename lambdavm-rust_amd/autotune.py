@@ -12,7 +12,8 @@ hvm_hip_cfg):
                    that the output bag redistributes the busy waves' redexes;
   * turn_steps   — a fixed share interval (steps per turn) instead of the
                    scheduler's choice;
-  * flags        — HVM_HIP_LOCAL_NET (per-wave LDS nets) or not.
+  * flags        — HVM_HIP_LOCAL_NET (per-wave LDS nets) and/or
+                   HVM_HIP_COMPILED_CALLS (def templates pre-reduced at load).
 The per-lane redex deque (HVM_DEQ) and the LDS net sizes are compile-time
 constants of the kernels and are not searched here.
 
@@ -89,7 +90,7 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
         ("waves", [waves_def * k // 8 for k in (2, 4, 6, 7)] + [0]),
         ("idle_pct", [10, 25, 75, 101, 0]),
         ("turn_steps", [96, 512, 4096, 0]),
-        ("flags", [H.LOCAL_NET, 0]),
+        ("flags", [H.COMPILED_CALLS, H.LOCAL_NET, H.LOCAL_NET | H.COMPILED_CALLS, 0]),
     ]
     best, t_best = dict(base), t_def
     rows = [{"cfg": dict(base), "ms": round(t_def, 3)}]

@@ -121,6 +121,20 @@ std::vector<uint32_t> to_buffer(const Book& book);
 // a def exceeds the loader limits (src/hvm.c:1432-1440).
 bool load_buffer(const uint32_t* buf, size_t n_words, Book& out, std::string& err);
 
+// Compiled CALLs (hvm_spec.cpp) ---------------------------------------------
+// Static interactions folded into one def's template: inlined CALLs and
+// ANNIs, added to the counters at every expansion, and the canonical bytes of
+// one expansion with them (the unspecialised def's own plus the static ones),
+// which replace the specialised template's own count.
+struct SpecCounts {
+  uint32_t call = 0, anni = 0, bytes = 0;
+};
+// Every def's template reduced at load (REF~node CALLs inlined, same-tag node
+// pairs annihilated, vars linked) within `node_budget` live nodes and
+// `vars_max` vars; `counts[f]` receives def f's static counts.  Defs that do
+// not reduce keep their template and zero counts.
+Book specialize_book(const Book& book, uint32_t node_budget, uint32_t vars_max, std::vector<SpecCounts>& counts);
+
 // Printers ------------------------------------------------------------------
 std::string show_numb(uint32_t numb);      // Rust Numb::show (canonical form)
 std::string c_show_numb(uint32_t numb);    // C pretty_print_numb (stdout contract)

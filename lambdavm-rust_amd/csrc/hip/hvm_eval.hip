@@ -112,6 +112,8 @@ struct Lane {
   u32 bot, top;                 // deque ring counters
   u64 bytes;
   u32 spills;
+  u64 spec;                     // static CALLs (low half) and ANNIs (high half)
+                                // of the compiled defs this lane expanded
 };
 
 // Wave-wide interaction counts (wave-uniform, kept in scalar registers).
@@ -217,7 +219,12 @@ __device__ __forceinline__ void push_redex(WaveLds& L, LocalNet* N, Lane& ln, co
       if (pos < HVM_LSPILL) A.lspill[(u64)blockIdx.x * HVM_LSPILL + pos] = old;
       else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     } else {
-      u32 pos = atomicAdd(&A.ctl->n_out, 1u);
+      // one bag reservation for all lanes spilling at this push
+      const u64 m = __ballot(1);
+      const u32 leader = (u32)__ffsll((long long)m) - 1u;
+      u32 base = 0;
+      if (__lane_id() == leader) base = atomicAdd(&A.ctl->n_out, (u32)__popcll(m));
+      const u32 pos = __shfl(base, (int)leader, HVM_WAVE) + rank_in(m);
       if (pos < A.bag_cap) A.bag[A.ctl->parity ^ 1u][pos] = old;
       else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     }
@@ -511,6 +518,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   wc.leaks = wc.saves = 0;
   ln.bytes = 0;
   ln.spills = 0;
+  ln.spec = 0;
   u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0;
 
   if (LOC) {
@@ -719,7 +727,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     }
     const HvmDevDef& d = dx.h;
     if (has && rule == HVM_CALL) {
-      if (d.safe && hvm_get_tag(b) == HVM_DUP) copy = true;
+      if ((d.safe & 1u) && hvm_get_tag(b) == HVM_DUP) copy = true;
       else { nn = d.nlen; nv = d.vlen; }
     }
 
@@ -1016,7 +1024,11 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         switch (rule) {
           case HVM_CALL:
             if (copy) { kk = HVM_K_CALL_COPY; kb = 32; }
-            else { kk = HVM_K_CALL; kb = d.call_bytes; }
+            else {
+              kk = HVM_K_CALL; kb = d.call_bytes;
+              // a compiled def's static interactions (zero for plain defs)
+              ln.spec += (u64)((d.safe >> 8) & 0xFFu) | ((u64)(d.safe >> 16) << 32);
+            }
             break;
           case HVM_VOID: kk = HVM_K_VOID; break;
           case HVM_ERAS: kk = HVM_K_ERAS; kb = 32; break;
@@ -1106,7 +1118,12 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     u64 s_itrs = 0;
     u64 kinds[HVM_K_KINDS];
 #pragma unroll
-    for (int k = 0; k < HVM_K_KINDS; ++k) { kinds[k] = wc.kind[k]; s_itrs += wc.kind[k]; }
+    for (int k = 0; k < HVM_K_KINDS; ++k) kinds[k] = wc.kind[k];
+    // per lane each half stays below 2^32 for turns of up to 2^20 steps
+    kinds[HVM_K_CALL] += wave_sum64(ln.spec & 0xFFFFFFFFull);
+    kinds[HVM_K_ANNI] += wave_sum64(ln.spec >> 32);
+#pragma unroll
+    for (int k = 0; k < HVM_K_KINDS; ++k) s_itrs += kinds[k];
     u64 s_bytes = wave_sum64(ln.bytes);
     u64 s_spill = wave_sum64(ln.spills);
     defers = wave_sum64(defers);    // per-lane counts; steps/probes/lane_steps are wave-uniform

@@ -176,6 +176,18 @@ struct hvm_hip_ctx {
     if (!buf) return fail("failed to load book: NULL buffer");
     if (!hvm::load_buffer(buf, n_words, b, err)) return fail(err);
     if (b.defs.empty()) return fail("failed to load book: no defs");
+    // Compiled CALLs: templates pre-reduced at load, with the static
+    // interactions of each def counted at its every expansion.
+    std::vector<hvm::SpecCounts> spec;
+    if (cfg.flags & HVM_HIP_COMPILED_CALLS) {
+      for (const hvm::Def& d : b.defs)
+        if (d.node.size() > HVM_GPU_DEF_MAX || d.vars > HVM_GPU_DEF_MAX)
+          return fail("def '" + d.name + "' exceeds the evaluator's per-def limit of " +
+                      std::to_string(HVM_GPU_DEF_MAX) + " nodes/vars");
+      hvm::Book sb = hvm::specialize_book(b, HVM_HIP_SPEC_NODES, HVM_GPU_DEF_MAX, spec);
+      // the printer names REFs by the original book; names are unchanged
+      b = std::move(sb);
+    }
     std::vector<HvmDevDef> defs(b.defs.size());
     std::vector<u64> pairs;
     for (size_t f = 0; f < b.defs.size(); ++f) {
@@ -208,6 +220,11 @@ struct hvm_hip_ctx {
       x.root = d.root;
       x.safe = d.safe ? 1 : 0;
       x.call_bytes = 8 * x.nlen + 4 * x.vlen + 8 * (x.rlen + 1);
+      if (!spec.empty() && (spec[f].call || spec[f].anni)) {
+        // bit 0 safe, bits 8-15 static CALLs, bits 16-31 static ANNIs
+        x.safe |= (spec[f].call << 8) | (spec[f].anni << 16);
+        x.call_bytes = spec[f].bytes;
+      }
     }
     if (pairs.empty()) pairs.push_back(0);
     // records with the first template pairs inline (HvmDevDefX)
@@ -328,9 +345,9 @@ struct hvm_hip_ctx {
     c.n_in = 1;
     c.in_head = 1;
     c.parity = 0;
-    c.fixed_steps = cfg.turn_steps;
+    c.fixed_steps = std::min(cfg.turn_steps, 1u << 20);
     c.idle_quit = turn_args().idle_quit;
-    c.max_steps = cfg.turn_steps ? cfg.turn_steps : hvm_turn_policy_host(1, nwaves);
+    c.max_steps = cfg.turn_steps ? std::min(cfg.turn_steps, 1u << 20) : hvm_turn_policy_host(1, nwaves);
     c.turns = 0;
     c.done = 0;
     host_ctl = c;
@@ -879,9 +896,9 @@ struct hvm_hip_ctx {
     memset(&c, 0, sizeof c);
     c.n_in = n;
     c.in_head = std::min<u32>(n, nwaves * HVM_WAVE);
-    c.fixed_steps = cfg.turn_steps;
+    c.fixed_steps = std::min(cfg.turn_steps, 1u << 20);
     c.idle_quit = turn_args().idle_quit;
-    c.max_steps = cfg.turn_steps ? cfg.turn_steps : hvm_turn_policy_host(n, nwaves);
+    c.max_steps = cfg.turn_steps ? std::min(cfg.turn_steps, 1u << 20) : hvm_turn_policy_host(n, nwaves);
     host_ctl = c;
     if (!ck(hipMemcpyAsync(ctl, &host_ctl, sizeof(HvmCtl), hipMemcpyHostToDevice, stream), "boot ctl")) return false;
     return ck(hipStreamSynchronize(stream), "boot sync");
@@ -1094,6 +1111,37 @@ extern "C" int hvm_front_compile(const char* code, uint32_t** words, size_t* n_w
     if (err) *err = dup_cstr(ex.what());
     return 1;
   }
+}
+
+extern "C" int hvm_front_specialize(const uint32_t* words, size_t n_words, uint32_t node_budget,
+                                    uint32_t** out_words, size_t* out_n, uint32_t** counts, char** err) {
+  if (out_words) *out_words = nullptr;
+  if (out_n) *out_n = 0;
+  if (counts) *counts = nullptr;
+  if (err) *err = nullptr;
+  if (!words || !out_words || !out_n || !counts || node_budget > HVM_GPU_DEF_MAX) return HVM_HIP_E_ARG;
+  hvm::Book b;
+  std::string e;
+  if (!hvm::load_buffer(words, n_words, b, e)) {
+    if (err) *err = dup_cstr(e.c_str());
+    return HVM_HIP_E_BOOK;
+  }
+  std::vector<hvm::SpecCounts> spec;
+  hvm::Book sb = hvm::specialize_book(b, node_budget, HVM_GPU_DEF_MAX, spec);
+  std::vector<uint32_t> w = hvm::to_buffer(sb);
+  uint32_t* p = (uint32_t*)malloc(w.size() * 4 + 4);
+  uint32_t* c = (uint32_t*)malloc(spec.size() * 12 + 4);
+  if (!p || !c) { free(p); free(c); return HVM_HIP_E_ARG; }
+  memcpy(p, w.data(), w.size() * 4);
+  for (size_t f = 0; f < spec.size(); ++f) {
+    c[3 * f] = spec[f].call;
+    c[3 * f + 1] = spec[f].anni;
+    c[3 * f + 2] = spec[f].bytes;
+  }
+  *out_words = p;
+  *out_n = w.size();
+  *counts = c;
+  return HVM_HIP_OK;
 }
 
 extern "C" int hvm_front_canonical(const char* printed, char** out, char** err) {

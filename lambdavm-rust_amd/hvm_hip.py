@@ -28,6 +28,8 @@ NO_READBACK = 1
 FENCE_RELEASE = 2
 PROFILE = 4
 LOCAL_NET = 8
+COMPILED_CALLS = 16
+SPEC_NODES = 64
 
 # Scheduler (include/hvm_hip.h)
 SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
@@ -79,6 +81,10 @@ def lib() -> C.CDLL:
     L.hvm_front_compile.restype = C.c_int
     L.hvm_front_canonical.argtypes = [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
     L.hvm_front_canonical.restype = C.c_int
+    L.hvm_front_specialize.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, C.c_uint32,
+                                       C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_size_t),
+                                       C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_void_p)]
+    L.hvm_front_specialize.restype = C.c_int
     L.hvm_hip_free.argtypes = [C.c_void_p]
     L.hvm_hip_free.restype = None
     L.hvm_hip_create.argtypes = [C.POINTER(_Cfg)]
@@ -134,6 +140,26 @@ def compile_book(code: str) -> List[int]:
     out = words[: n.value]
     L.hvm_hip_free(C.cast(words, C.c_void_p))
     return list(out)
+
+
+def specialize(words, node_budget: int = SPEC_NODES):
+    """The load-time specialisation of COMPILED_CALLS on a wire buffer:
+    returns (specialised words, [(static CALLs, static ANNIs, bytes)] per def)."""
+    L = lib()
+    arr = words_array(words)
+    out = C.POINTER(C.c_uint32)()
+    n = C.c_size_t()
+    cnt = C.POINTER(C.c_uint32)()
+    err = C.c_void_p()
+    rc = L.hvm_front_specialize(arr, len(words), node_budget, C.byref(out), C.byref(n), C.byref(cnt), C.byref(err))
+    if rc != 0:
+        raise HvmError(_take_str(err.value) or f"specialize failed ({rc})")
+    new = list(out[: n.value])
+    ndefs = new[0]
+    counts = [tuple(cnt[3 * f: 3 * f + 3]) for f in range(ndefs)]
+    L.hvm_hip_free(C.cast(out, C.c_void_p))
+    L.hvm_hip_free(C.cast(cnt, C.c_void_p))
+    return new, counts
 
 
 def canonical(printed: str) -> str:

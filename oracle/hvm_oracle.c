@@ -37,6 +37,8 @@ typedef struct {
   hvm_oracle_stats* st;
   uint64_t live_nodes;
   int oom;
+  /* optional: expansions per fid (hvm_oracle_run_calls) */
+  uint64_t* calls; uint32_t ncalls;
 } ONet;
 
 /* ----------------------------------------------------------------------- */
@@ -207,6 +209,7 @@ static int interact_call(ONet* n, Port a, Port b) {
   }
   link_ports(n, adjust_port(n, d->root), b);
   count(n, HVM_ORC_CALL, 8ull * d->nlen + 4ull * d->vlen + 8ull * (d->rlen + 1));
+  if (n->calls && fid < n->ncalls) n->calls[fid] += 1;
   return 1;
 }
 
@@ -510,7 +513,8 @@ static void free_net(ONet* n) {
   free(n->node); free(n->vars); free(n->hbag); free(n->rbag); free(n->nloc); free(n->vloc);
 }
 
-int hvm_oracle_run(const uint32_t* book_buf, size_t n_words, char** result, hvm_oracle_stats* st) {
+int hvm_oracle_run_calls(const uint32_t* book_buf, size_t n_words, char** result, hvm_oracle_stats* st,
+                         uint64_t* calls, uint32_t ncalls) {
   hvm_oracle_stats local;
   if (!st) st = &local;
   memset(st, 0, sizeof *st);
@@ -518,6 +522,9 @@ int hvm_oracle_run(const uint32_t* book_buf, size_t n_words, char** result, hvm_
   ONet n;
   memset(&n, 0, sizeof n);
   n.st = st;
+  n.calls = calls;
+  n.ncalls = calls ? ncalls : 0;
+  if (calls) memset(calls, 0, sizeof(uint64_t) * ncalls);
   n.nput = 1;
   n.vput = 1;
   if (!book_buf || !load_book(&n, book_buf, n_words) || n.ndefs == 0) { free_net(&n); return 1; }
@@ -534,6 +541,10 @@ int hvm_oracle_run(const uint32_t* book_buf, size_t n_words, char** result, hvm_
   if (result) *result = text; else free(text);
   free_net(&n);
   return 0;
+}
+
+int hvm_oracle_run(const uint32_t* book_buf, size_t n_words, char** result, hvm_oracle_stats* st) {
+  return hvm_oracle_run_calls(book_buf, n_words, result, st, NULL, 0);
 }
 
 void hvm_oracle_free(void* p) { free(p); }

@@ -66,6 +66,11 @@ typedef struct {
  * performs IO (out of scope), 3 on allocation failure. */
 int hvm_oracle_run(const uint32_t* book_buf, size_t n_words, char** result, hvm_oracle_stats* st);
 
+/* As hvm_oracle_run, also counting the CALLs that expand each def: calls[fid]
+ * for fid < ncalls (a checker for def-level transformations of a book). */
+int hvm_oracle_run_calls(const uint32_t* book_buf, size_t n_words, char** result, hvm_oracle_stats* st,
+                         uint64_t* calls, uint32_t ncalls);
+
 void hvm_oracle_free(void* p);
 
 #ifdef __cplusplus

@@ -42,6 +42,7 @@ class OracleResult:
     normalizes: int = 0
     peak_nodes: int = 0
     seconds: float = 0.0
+    calls: Optional[list] = None   # expansions per fid (run_oracle(..., calls=True))
 
 
 _lib = None
@@ -56,18 +57,28 @@ def lib():
         L.hvm_oracle_run.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_void_p),
                                      C.POINTER(_Stats)]
         L.hvm_oracle_run.restype = C.c_int
+        L.hvm_oracle_run_calls.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_void_p),
+                                           C.POINTER(_Stats), C.POINTER(C.c_uint64), C.c_uint32]
+        L.hvm_oracle_run_calls.restype = C.c_int
         L.hvm_oracle_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
 
-def run_oracle(words) -> OracleResult:
+def run_oracle(words, calls: bool = False) -> OracleResult:
     L = lib()
     arr = (C.c_uint32 * len(words))(*words)
     res = C.c_void_p()
     st = _Stats()
-    rc = L.hvm_oracle_run(arr, len(words), C.byref(res), C.byref(st))
+    ndefs = words[0] if words else 0
+    per_fid = (C.c_uint64 * max(ndefs, 1))()
+    if calls:
+        rc = L.hvm_oracle_run_calls(arr, len(words), C.byref(res), C.byref(st), per_fid, ndefs)
+    else:
+        rc = L.hvm_oracle_run(arr, len(words), C.byref(res), C.byref(st))
     out = OracleResult(status=rc)
+    if calls:
+        out.calls = list(per_fid[:ndefs])
     if res.value:
         out.result = C.string_at(res.value).decode()
         L.hvm_oracle_free(res.value)

@@ -145,7 +145,7 @@ def test_autotuner_keeps_results(hip):
     w = T.batched(seed=5, n=12)
     rep = T.tune(w, repeats=1, log=print, base_cfg={},
                  space=[("waves", [512, 0]), ("idle_pct", [101, 0]), ("turn_steps", [96, 0]),
-                        ("flags", [hip.LOCAL_NET, 0])])
+                        ("flags", [hip.COMPILED_CALLS, hip.LOCAL_NET, 0])])
     assert rep["itrs"] > 0
     assert rep["tuned_ms"] <= rep["default_ms"]
     assert any(m.get("cfg") == rep["config"] for m in rep["measurements"])
