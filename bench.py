@@ -10,8 +10,9 @@ Steps are timed with the wall clock around K steps; the dominant kernel
 library.  Parity (ITRS and normal form vs the reference run-c golden record)
 is checked on a warmup step outside the timed region.  The timed steps
 interpret the book as run-cu does; the compiled-CALL mode (def templates
-pre-reduced at load, same ITRS) is checked and timed after them and reported
-in "mode_compare" (`--compiled` swaps the two).
+pre-reduced at load, same ITRS) and the 64-bit-port evaluator are checked
+and timed after them and reported in "mode_compare" (`--compiled` swaps the
+first two).
 
 Prints one JSON line (see the contract in the repository instructions).
 """
@@ -201,27 +202,32 @@ def main():
         compare[other + "_mips"] = round(runs[0].itrs * args.steps / (t3 - t2) / 1e6, 2)
     ctx.close()
 
-    # the other evaluation mode (interpreted / compiled CALLs), after the timed
-    # region: parity against the golden record, then the same step count
-    variant = None
+    # the other evaluation modes (interpreted / compiled CALLs, and the
+    # 64-bit-port evaluator), after the timed region: parity against the
+    # golden record, then the same step count
+    variant = []
     if not args.no_sched_compare:
-        vflag = 0 if args.compiled else hvm_hip.COMPILED_CALLS
-        with hvm_hip.Context(time_limit_s=600, flags=vflag) as vctx:
-            vctx.load(words)
-            rv = vctx.eval()
-        with hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK | vflag) as vctx:
-            vctx.load(words)
-            use(vctx, args.sched)
-            vctx.eval()
-            t4 = time.perf_counter()
-            v_runs = [vctx.eval() for _ in range(args.steps)]
-            t5 = time.perf_counter()
-        variant = {"mode": "interpreted" if args.compiled else "compiled_calls",
-                   "value": round(v_runs[0].itrs * args.steps / (t5 - t4) / 1e6, 2),
-                   "ms_per_step": round((t5 - t4) / args.steps * 1e3, 3),
-                   "itrs_match": gcase is not None and rv.itrs == gcase["itrs"],
-                   "nf_match": gcase is not None and
-                   hashlib.sha256(rv.canonical.encode()).hexdigest() == gcase["nf_sha256"]}
+        modes = [("interpreted" if args.compiled else "compiled_calls",
+                  0 if args.compiled else hvm_hip.COMPILED_CALLS),
+                 ("wide_ports", hvm_hip.WIDE)]
+        for name, vflag in modes:
+            with hvm_hip.Context(time_limit_s=600, flags=vflag) as vctx:
+                vctx.load(words)
+                rv = vctx.eval()
+            with hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK | vflag) as vctx:
+                vctx.load(words)
+                if not vflag & hvm_hip.WIDE:
+                    use(vctx, args.sched)
+                vctx.eval()
+                t4 = time.perf_counter()
+                v_runs = [vctx.eval() for _ in range(args.steps)]
+                t5 = time.perf_counter()
+            variant.append({"mode": name,
+                            "value": round(v_runs[0].itrs * args.steps / (t5 - t4) / 1e6, 2),
+                            "ms_per_step": round((t5 - t4) / args.steps * 1e3, 3),
+                            "itrs_match": gcase is not None and rv.itrs == gcase["itrs"],
+                            "nf_match": gcase is not None and
+                            hashlib.sha256(rv.canonical.encode()).hexdigest() == gcase["nf_sha256"]})
 
     itrs = runs[0].itrs
     assert all(r.itrs == itrs for r in runs), "ITRS changed between steps"

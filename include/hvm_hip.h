@@ -40,8 +40,9 @@ void hvm_hip(uint32_t* book_buffer);
 /* ------------------------------------------------------------------------ */
 
 typedef struct hvm_hip_cfg {
-  uint64_t node_cap;    /* node slots in the net (0: 2^28)                    */
-  uint64_t vars_cap;    /* var slots in the net (0: 2^28)                     */
+  uint64_t node_cap;    /* node slots in the net (0: 2^28; at most 2^29, or
+                           2^32 - 2 with HVM_HIP_WIDE)                        */
+  uint64_t vars_cap;    /* var slots in the net (0: 2^28; limits as node_cap) */
   uint32_t bag_cap;     /* redex-bag entries per bag (0: 2^24)                */
   uint32_t waves;       /* evaluator waves (0: fill every CU)                 */
   uint32_t turn_steps;  /* steps per turn (0: adaptive; at most 2^20)         */
@@ -73,6 +74,14 @@ typedef struct hvm_hip_cfg {
  * kind[], so ITRS, kind[] and the normal form equal run-c's.  Fewer nodes are
  * created and taken per interaction counted. */
 #define HVM_HIP_COMPILED_CALLS 16u
+/* 64-bit ports (paper/HVM2.typst:1212-1222, the reference's planned
+ * extension): ports of 64 bits, nodes of two 64-bit halves, slots indexed by
+ * 32-bit integers, so node_cap and vars_cap may reach 2^32 - 2 (64 GiB of
+ * nodes, 32 GiB of vars) instead of 2^29.  Numbers keep the reference's
+ * 29-bit encoding: results, ITRS and kind[] equal run-c's.  A net of its own
+ * with its own kernel (one redex per lane per step); no IO driver, no batched
+ * nets, no local nets, plain scheduling only. */
+#define HVM_HIP_WIDE 32u
 #define HVM_HIP_SPEC_NODES 64u
 
 /* Interaction kinds, in the order of hvm_hip_out.kind[]. */

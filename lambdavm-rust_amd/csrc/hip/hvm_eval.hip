@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hvm_dev.hpp"
+#include "hvm_wave.hpp"
 
 typedef unsigned long long u64;
 typedef uint32_t u32;
@@ -55,44 +56,6 @@ __device__ __forceinline__ u32 vars_xchg(const HvmTurnArgs& A, u32 val, u32 x) {
 }
 __device__ __forceinline__ void vars_put(const HvmTurnArgs& A, u32 val, u32 x) {
   __hip_atomic_store(&A.vars[vars_slot(A, val)], x, __ATOMIC_RELAXED, AGENT);
-}
-
-// ---------------------------------------------------------------------------
-// Wave primitives
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ u32 lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ u32 rank_in(u64 mask) {
-  return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
-}
-
-// Exclusive prefix sum over the wave; *total receives the wave sum.
-__device__ __forceinline__ u32 wave_scan(u32 x, u32* total) {
-  u32 v = x;
-  u32 lane = lane_id();
-#pragma unroll
-  for (u32 d = 1; d < HVM_WAVE; d <<= 1) {
-    u32 t = __shfl_up(v, d, HVM_WAVE);
-    if (lane >= d) v += t;
-  }
-  *total = __shfl(v, HVM_WAVE - 1, HVM_WAVE);
-  return v - x;
-}
-
-__device__ __forceinline__ u32 wave_max(u32 x) {
-#pragma unroll
-  for (u32 d = 1; d < HVM_WAVE; d <<= 1) {
-    u32 t = __shfl_xor(x, d, HVM_WAVE);
-    x = x > t ? x : t;
-  }
-  return x;
-}
-
-__device__ __forceinline__ u64 wave_sum64(u64 x) {
-#pragma unroll
-  for (u32 d = 1; d < HVM_WAVE; d <<= 1) x += __shfl_xor(x, d, HVM_WAVE);
-  return x;
 }
 
 // ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@
 #include "../../../include/hvm_hip.h"
 #include "../front/hvm_front.hpp"
 #include "hvm_dev.hpp"
+#include "hvm_wide.hpp"
 
 extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s);
 extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s);
@@ -74,6 +75,8 @@ struct hvm_hip_ctx {
   u64* d_pairs = nullptr;
   u32 ndefs = 0;
   bool loaded = false;
+  // the 64-bit-port evaluator's net (HVM_HIP_WIDE), instead of the above
+  HvmWide* wide = nullptr;
   // per-eval bookkeeping
   double kernel_ms = 0, launch_ms = 0;
   u32 turns = 0, normalizes = 0, launches = 0;
@@ -95,6 +98,20 @@ struct hvm_hip_ctx {
     cus = prop.multiProcessorCount;
     uint64_t ncap = cfg.node_cap ? cfg.node_cap : (1ull << 28);
     uint64_t vcap = cfg.vars_cap ? cfg.vars_cap : (1ull << 28);
+    if (cfg.flags & HVM_HIP_WIDE) {
+      // 64-bit ports: capacities up to 2^32 - 2 slots, a net of its own
+      if (cfg.flags & HVM_HIP_LOCAL_NET) return fail("HVM_HIP_WIDE and HVM_HIP_LOCAL_NET exclude each other");
+      if (!ck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+      wide = new HvmWide();
+      wide->turn_steps = cfg.turn_steps;
+      wide->idle_pct = cfg.idle_pct ? cfg.idle_pct : 25u;
+      if (!wide->create(ncap, vcap, cfg.bag_cap ? cfg.bag_cap : (1u << 24), cfg.waves, cus, stream))
+        return fail(wide->err);
+      node_cap = wide->a.node_cap;
+      vars_cap = wide->a.vars_cap;
+      nwaves = wide->a.nwaves;
+      return true;
+    }
     if (ncap > (1ull << 29) || vcap > (1ull << 29)) return fail("node_cap/vars_cap exceed 2^29");
     const bool local = (cfg.flags & HVM_HIP_LOCAL_NET) != 0;
     if (local && (ncap > HVM_LOCAL_BASE || vcap > HVM_LOCAL_BASE))
@@ -151,6 +168,7 @@ struct hvm_hip_ctx {
   }
 
   void release() {
+    if (wide) { wide->destroy(); delete wide; wide = nullptr; }
     if (node) hipFree(node);
     if (vars) hipFree(vars);
     if (bag[0]) hipFree(bag[0]);
@@ -825,8 +843,56 @@ struct hvm_hip_ctx {
     return fail("result too large to read back");
   }
 
+  // The 64-bit-port evaluator: boot, normalise, read back (no IO driver).
+  int eval_wide(hvm_hip_out* out) {
+    double t0 = now_s();
+    const double dl = cfg.time_limit_s ? t0 + cfg.time_limit_s : 0;
+    HvmDevStats s;
+    memset(&s, 0, sizeof s);
+    u32 wturns = 0, wlaunches = 0;
+    int rc = wide->normalize(d_defs, d_pairs, ndefs, dl, s, wturns, wlaunches);
+    double t_eval = now_s();
+    if (out) {
+      memset(out, 0, sizeof *out);
+      out->itrs = s.itrs;
+      for (int k = 0; k < HVM_HIP_KINDS; ++k) out->kind[k] = s.kind[k];
+      out->bytes = s.bytes;
+      out->steps = s.steps; out->spills = s.spills; out->defers = s.defers; out->retries = s.retries;
+      out->eval_ms = (t_eval - t0) * 1e3;
+      out->turns = wturns;
+      out->launches = wlaunches;
+      out->waves = nwaves;
+      out->normalizes = 1;
+    }
+    if (rc != HVM_HIP_OK) {
+      fail(wide->err);
+      if (out) {
+        const size_t at = wide->err.find("flags 0x");
+        out->err = at == std::string::npos ? 0u : (u32)strtoul(wide->err.c_str() + at + 8, nullptr, 16);
+      }
+      return rc;
+    }
+    if (!(cfg.flags & HVM_HIP_NO_READBACK)) {
+      std::vector<std::string> names;
+      for (const hvm::Def& d : book.defs) names.push_back(d.name);
+      std::string text;
+      if (!wide->print_root(names, text)) { fail(wide->err); return HVM_HIP_E_DEVICE; }
+      if (out) {
+        out->wall_s = now_s() - t0;
+        out->result = dup_cstr(text);
+        std::string canon;
+        try { canon = hvm::canonical_tree_text(text); } catch (std::exception& ex) { canon = "<unparsable: " + std::string(ex.what()) + ">"; }
+        out->canonical = dup_cstr(canon);
+      }
+    } else if (out) {
+      out->wall_s = t_eval - t0;
+    }
+    return HVM_HIP_OK;
+  }
+
   int eval(hvm_hip_out* out) {
     if (!loaded) { fail("no book loaded"); return HVM_HIP_E_ARG; }
+    if (wide) return eval_wide(out);
     kernel_ms = 0;
     launch_ms = 0;
     turns = 0;
@@ -926,6 +992,7 @@ struct hvm_hip_ctx {
   }
 
   int eval_batch(const u32* entries, u32 n, const uint64_t* est, hvm_hip_out* outs, hvm_hip_out* total) {
+    if (wide) { fail("batched evaluation runs on the 32-bit evaluator (no HVM_HIP_WIDE)"); return HVM_HIP_E_ARG; }
     if (!loaded) { fail("no book loaded"); return HVM_HIP_E_ARG; }
     if (!entries || !outs || n == 0) { fail("empty batch"); return HVM_HIP_E_ARG; }
     for (u32 j = 0; j < n; ++j) {
@@ -1013,11 +1080,16 @@ extern "C" int hvm_hip_load(hvm_hip_ctx* ctx, const uint32_t* book_buf, size_t n
 extern "C" int hvm_hip_set_scheduler(hvm_hip_ctx* ctx, uint32_t mode, uint32_t seed, uint32_t explore_pct,
                                      const uint16_t* params, uint32_t trace_cap) {
   if (!ctx) return HVM_HIP_E_ARG;
+  if (ctx->wide) {
+    // the wide evaluator runs fixed-length turns ended by idle quitting
+    ctx->last_error = "scheduler modes apply to the 32-bit evaluator (no HVM_HIP_WIDE)";
+    return mode == HVM_HIP_SCHED_PLAIN ? HVM_HIP_OK : HVM_HIP_E_ARG;
+  }
   return ctx->set_scheduler(mode, seed, explore_pct, params, trace_cap) ? HVM_HIP_OK : HVM_HIP_E_ARG;
 }
 
 extern "C" int hvm_hip_read_trace(hvm_hip_ctx* ctx, float* recs, uint32_t cap, uint32_t* n_recs) {
-  if (!ctx) return HVM_HIP_E_ARG;
+  if (!ctx || ctx->wide) return HVM_HIP_E_ARG;
   return ctx->read_trace(recs, cap, n_recs);
 }
 

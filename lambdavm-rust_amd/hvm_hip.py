@@ -29,6 +29,7 @@ FENCE_RELEASE = 2
 PROFILE = 4
 LOCAL_NET = 8
 COMPILED_CALLS = 16
+WIDE = 32
 SPEC_NODES = 64
 
 # Scheduler (include/hvm_hip.h)

@@ -112,6 +112,35 @@ def sum_tree(depth: int = 20) -> Tuple[str, int]:
     return code, (1 << depth) % U24
 
 
+def tree_gen(depth: int = 30) -> Tuple[str, str]:
+    """A full binary tree of CON nodes with 2^depth leaves `1`, built
+    recursively: its normal form alone holds 2^depth - 1 nodes (depth 30 does
+    not fit a net of 2^29 slots).  Returns the code and, for depth <= 16, the
+    printed normal form (else "")."""
+    code = f"""
+@main = r & @gen ~ ({depth} r)
+@gen = (?((1 @gen_s) r) r)
+@gen_s = ({{a b}} (x y))
+  &! @gen ~ (a x)
+  &! @gen ~ (b y)
+"""
+    nf = ""
+    if depth <= 16:
+        nf = "1"
+        for _ in range(depth):
+            nf = f"({nf} {nf})"
+    return code, nf
+
+
+def tree_gen_kinds(depth: int) -> dict:
+    """Interaction counts of tree_gen(depth) by kind, as closed forms of the
+    depth (ITRS = 13 * 2^depth - 7); tests pin them against the oracle."""
+    n = 1 << depth
+    k = {"CALL": 3 * n - 1, "CALL_COPY": 0, "VOID": 2 * n - 1, "ERAS": n - 1, "ANNI": 5 * n - 3,
+         "COMM": 0, "OPER1": 0, "OPER2": 0, "SWIT0": n, "SWITN": n - 1}
+    return k
+
+
 BITONIC_DEFS = """
 @sort = (?((@sort_z @sort_s) (s (t r))) (s (t r)))
 @sort_z = (* (t t))

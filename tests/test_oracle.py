@@ -94,3 +94,22 @@ def test_reference_itrs_is_schedule_invariant(hip, orc):
     w = hip.compile_book(code)
     counts = {orc.run_reference(w, t).itrs for t in (1, 2, 4, 8)}
     assert len(counts) == 1
+
+
+def test_tree_gen_closed_forms(hip, orc):
+    """tree_gen's per-kind counts and normal form as closed forms of the depth,
+    pinned against the oracle at depths 0-12 and against the reference run-c
+    (where built) at depths 4 and 10: the GPU capacity test uses them at
+    depth 30, past what a 2^29-slot net holds."""
+    import programs as P
+    for d in range(13):
+        code, nf = P.tree_gen(d)
+        o = orc.run_oracle(hip.compile_book(code))
+        assert o.status == 0
+        assert o.kinds == P.tree_gen_kinds(d), d
+        assert o.itrs == 13 * 2 ** d - 7
+        assert o.result == nf
+    if orc.have_reference(1):
+        for d in (4, 10):
+            ref = orc.run_reference(hip.compile_book(P.tree_gen(d)[0]), threads=1)
+            assert ref.itrs == 13 * 2 ** d - 7
