@@ -202,6 +202,8 @@ struct HvmTurnArgs {
 #define HVM_TURN_PROFILE 2u
 // Run the local-net evaluator (node/var allocation in LDS).
 #define HVM_TURN_LOCAL 4u
+// Fast CALL paths of compiled CALLs (HBM evaluator only).
+#define HVM_TURN_FAST 8u
 
 struct HvmGatherArgs {
   const unsigned long long* node;

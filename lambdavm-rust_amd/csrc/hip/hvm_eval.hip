@@ -77,6 +77,7 @@ struct Lane {
   u32 spills;
   u64 spec;                     // static CALLs (low half) and ANNIs (high half)
                                 // of the compiled defs this lane expanded
+  u32 fsw0, fswn;               // SWITCHes performed inside fast CALLs
 };
 
 // Wave-wide interaction counts (wave-uniform, kept in scalar registers).
@@ -458,7 +459,14 @@ __device__ __forceinline__ void put_link(u32* pa, u32* pb, u32& np, u32 p1, u32 
 // Steps a wave may spend draining its local net before giving up.
 #define HVM_EVAC_CAP (1u << 16)
 
-template <bool LOC>
+// FAST (compiled CALLs): a CALL whose caller's port is a CON node takes that
+// node with the def record and performs the ANNI between it and the def's
+// root node (a CON) instead of creating the root; if the root's first port is
+// a SWI node whose first port is a CON node and the caller passes a number,
+// the SWITCH and the ANNI of the branch pair happen too, without creating
+// either node (src/cmp.rs:106-170 fast SWITCH, 230-262 fast ANNI).  Each is
+// counted as the interaction it is.
+template <bool LOC, bool FAST>
 __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, LocalNet* N) {
   const u32 lane = lane_id();
   const u32 w = blockIdx.x;
@@ -482,6 +490,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   ln.bytes = 0;
   ln.spills = 0;
   ln.spec = 0;
+  ln.fsw0 = ln.fswn = 0;
   u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0;
 
   if (LOC) {
@@ -655,13 +664,15 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     u32 off_n = 0, off_v = 0;
     bool run = false;
     u64 NA = 0, NB = 0;
+    // fast CALL: the caller's CON node is taken in the def record's round trip
+    const bool fastb = FAST && EARLY && has && rule == HVM_CALL && hvm_get_tag(b) == HVM_CON;
     if (EARLY) {
       u32 tot;
       const u32 o = wave_scan(rule != HVM_CALL ? (nn | (nv << 16)) : 0u, &tot);
       off_n = o & 0xFFFFu; off_v = o >> 16;
       run = has && rule != HVM_CALL && off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
       NA = (run && take_a0) ? node_take(A, hvm_get_val(a)) : 0ull;
-      NB = (run && take_b0) ? node_take(A, hvm_get_val(b)) : 0ull;
+      NB = ((run && take_b0) || fastb) ? node_take(A, hvm_get_val(b)) : 0ull;
     }
     // A CALL's def record with its first template pairs (one 128-byte load).
     bool copy = false;
@@ -722,6 +733,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
             off_xn + xn <= HVM_TBL_X && off_xv + xv <= HVM_TBL_X;
     }
     if (has && !run) defers++;
+    if (fastb && !run && NB) node_put(A, hvm_get_val(b), NB);   // waits: the node goes back
     if (LOC && !evac && !__ballot(run)) {
       // the local net is too full for any redex here: drain it
       evac = true;
@@ -740,7 +752,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     // ---- P3: takes of the running lanes that have not taken yet (all of
     //      them in the local-net evaluator; a safe def's copy rule) ----
     const bool take_a = run && take_a0;
-    const bool take_b = run && (take_b0 || (rule == HVM_CALL && copy));
+    const bool take_b = run && (take_b0 || (rule == HVM_CALL && copy) || fastb);
     if (!EARLY) {
       NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
       NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
@@ -812,6 +824,20 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     }
 
     u32 B1 = hvm_get_fst(NB), B2 = hvm_get_snd(NB);
+    // fast CALL: the def's root node (a CON, template node 0) against the
+    // caller's CON; with a number there and the root's first port a SWI
+    // (node 1) whose first port is a CON (node 2) — the pre-order numbering
+    // of a def that switches on its first argument — the SWITCH too.
+    bool fa = false, fs = false;
+    const u32 r0 = 0, s0 = 1, c0 = 2;
+    u32 skip_m = 0;
+    const u64 R0 = dx.node[0], S0 = dx.node[1], C0 = dx.node[2];
+    if (FAST && run && fastb) {
+      fa = d.root == hvm_new_port(HVM_CON, 0);
+      fs = fa && hvm_get_fst(R0) == hvm_new_port(HVM_SWI, 1) && hvm_get_fst(S0) == hvm_new_port(HVM_CON, 2) &&
+           hvm_get_tag(B1) == HVM_NUM;
+      skip_m = fa ? (fs ? 7u : 1u) : 0u;
+    }
     bool oper1 = false;
     u32 cv = 0;
     if (run && rule == HVM_OPER) {
@@ -856,10 +882,25 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
           for (int k = 0; k < 4; ++k) { locs[k] = tn[c + k]; vals[k] = adjust_pair(tpre[c + k], tn, tv); }
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            if ((u32)(c + k) < d.nlen) net_put<LOC>(A, N, locs[k], vals[k]);
+            if ((u32)(c + k) < d.nlen && !((skip_m >> (c + k)) & 1u)) net_put<LOC>(A, N, locs[k], vals[k]);
         }
         for (u32 i = HVM_PRE_NODE; i < d.nlen; ++i)
           net_put<LOC>(A, N, tn[i], adjust_pair(A.pairs[d.node_off + i], tn, tv));
+        if (FAST && fastb) {
+          if (!fa) {
+            net_put<LOC>(A, N, hvm_get_val(b), NB);   // not a fast CALL: the caller's node goes back
+          } else {
+            // the slots of the nodes not created go back; s0's holds the
+            // SWITCH's (n-1, continuation) node when n > 0
+            const bool nz = fs && hvm_get_u24(hvm_get_val(B1)) != 0;
+            node_free<LOC>(A, N, tn[r0]);
+            if (fs) node_free<LOC>(A, N, tn[c0]);
+            if (fs && !nz) node_free<LOC>(A, N, tn[s0]);
+            if (nz)
+              net_put<LOC>(A, N, tn[s0], hvm_new_pair(hvm_new_port(HVM_NUM, hvm_new_u24(hvm_get_u24(hvm_get_val(B1)) - 1)),
+                                                      adjust_port(hvm_get_snd(S0), tn, tv)));
+          }
+        }
       } else if (rule == HVM_COMM) {
         var_put<LOC>(A, N, tv[0], HVM_NONE);
         var_put<LOC>(A, N, tv[1], HVM_NONE);
@@ -928,15 +969,32 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
               }
             }
             np = min(d.rlen, (u32)HVM_PRE_RBAG);
-            for (u32 i = np;; ++i) {
+            auto emit = [&](u32 x, u32 y) {
               if (np == HVM_LB) { link_batch<LOC>(L, N, ln, A, pa, pb, np); np = 0; }
-              if (i < d.rlen) {
-                u64 p = adjust_pair(A.pairs[d.rbag_off + i], tn, tv);
-                put_link(pa, pb, np, hvm_get_fst(p), hvm_get_snd(p));
+              put_link(pa, pb, np, x, y);
+            };
+            for (u32 i = np; i < d.rlen; ++i) {
+              u64 p = adjust_pair(A.pairs[d.rbag_off + i], tn, tv);
+              emit(hvm_get_fst(p), hvm_get_snd(p));
+            }
+            if (FAST && fs) {
+              // SWITCH on n: n = 0 links the zero branch to the continuation
+              // and erases the other; n > 0 erases the zero branch and
+              // passes (n-1, continuation) to the other
+              const u32 z = adjust_port(hvm_get_fst(C0), tn, tv), sc = adjust_port(hvm_get_snd(C0), tn, tv);
+              if (hvm_get_u24(hvm_get_val(B1)) == 0) {
+                emit(z, adjust_port(hvm_get_snd(S0), tn, tv));
+                emit(sc, hvm_new_port(HVM_ERA, 0));
               } else {
-                put_link(pa, pb, np, adjust_port(d.root, tn, tv), b);
-                break;
+                emit(z, hvm_new_port(HVM_ERA, 0));
+                emit(sc, hvm_new_port(HVM_CON, tn[s0]));
               }
+              emit(adjust_port(hvm_get_snd(R0), tn, tv), B2);
+            } else if (FAST && fa) {
+              emit(adjust_port(hvm_get_fst(R0), tn, tv), B1);
+              emit(adjust_port(hvm_get_snd(R0), tn, tv), B2);
+            } else {
+              emit(adjust_port(d.root, tn, tv), b);
             }
           }
           break;
@@ -991,6 +1049,18 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
               kk = HVM_K_CALL; kb = d.call_bytes;
               // a compiled def's static interactions (zero for plain defs)
               ln.spec += (u64)((d.safe >> 8) & 0xFFu) | ((u64)(d.safe >> 16) << 32);
+              if (FAST && fa) {
+                // the ANNI with the caller's node; with a SWITCH, that and
+                // the branch pair's ANNI
+                ln.spec += (fs ? 2ull : 1ull) << 32;
+                kb += fs ? 96u : 48u;
+                if (fs) {
+                  const bool z0 = hvm_get_u24(hvm_get_val(B1)) == 0;
+                  ln.fsw0 += z0 ? 1u : 0u;
+                  ln.fswn += z0 ? 0u : 1u;
+                  kb += z0 ? 32u : 40u;
+                }
+              }
             }
             break;
           case HVM_VOID: kk = HVM_K_VOID; break;
@@ -1085,6 +1155,10 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     // per lane each half stays below 2^32 for turns of up to 2^20 steps
     kinds[HVM_K_CALL] += wave_sum64(ln.spec & 0xFFFFFFFFull);
     kinds[HVM_K_ANNI] += wave_sum64(ln.spec >> 32);
+    if (FAST) {
+      kinds[HVM_K_SWIT0] += wave_sum64(ln.fsw0);
+      kinds[HVM_K_SWITN] += wave_sum64(ln.fswn);
+    }
 #pragma unroll
     for (int k = 0; k < HVM_K_KINDS; ++k) s_itrs += kinds[k];
     u64 s_bytes = wave_sum64(ln.bytes);
@@ -1126,7 +1200,13 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
 
 __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTurnArgs A) {
   __shared__ WaveLds L;
-  turn_body<false>(A, L, nullptr);
+  turn_body<false, false>(A, L, nullptr);
+}
+
+// Compiled CALLs (HVM_TURN_FAST): the same turn with the fast CALL paths.
+__global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_fast_kernel(HvmTurnArgs A) {
+  __shared__ WaveLds L;
+  turn_body<false, true>(A, L, nullptr);
 }
 
 // The local-net evaluator: the same turn with node/var allocation in LDS.
@@ -1134,7 +1214,7 @@ __global__ __launch_bounds__(HVM_WAVE, HVM_TURN_OCC) void hvm_turn_kernel(HvmTur
 __global__ __launch_bounds__(HVM_WAVE, 1) void hvm_turn_local_kernel(HvmTurnArgs A) {
   __shared__ WaveLds L;
   __shared__ LocalNet N;
-  turn_body<true>(A, L, &N);
+  turn_body<true, false>(A, L, &N);
 }
 
 // ---------------------------------------------------------------------------
@@ -1343,6 +1423,8 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
 extern "C" hipError_t hvm_launch_turn(const HvmTurnArgs* a, hipStream_t s) {
   if (a->flags & HVM_TURN_LOCAL)
     hipLaunchKernelGGL(hvm_turn_local_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
+  else if (a->flags & HVM_TURN_FAST)
+    hipLaunchKernelGGL(hvm_turn_fast_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
   else
     hipLaunchKernelGGL(hvm_turn_kernel, dim3(a->nwaves), dim3(HVM_WAVE), 0, s, *a);
   return hipGetLastError();

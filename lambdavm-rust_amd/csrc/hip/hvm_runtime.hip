@@ -395,6 +395,7 @@ struct hvm_hip_ctx {
     a.flags = (cfg.flags & HVM_HIP_FENCE_RELEASE) ? 0u : HVM_TURN_WAITCNT_FENCE;
     if (cfg.flags & HVM_HIP_PROFILE) a.flags |= HVM_TURN_PROFILE;
     if (cfg.flags & HVM_HIP_LOCAL_NET) a.flags |= HVM_TURN_LOCAL;
+    else if (cfg.flags & HVM_HIP_COMPILED_CALLS) a.flags |= HVM_TURN_FAST;
     a.lspill = lspill;
     // a turn ends early once idle_pct% of the waves ran out of work (default 25%)
     u32 pct = cfg.idle_pct ? (cfg.idle_pct > 100 ? 0u : cfg.idle_pct) : 25u;
@@ -1011,7 +1012,6 @@ struct hvm_hip_ctx {
     // admission budget: a quarter of the node capacity, leaving headroom for
     // the breadth-first growth of parallel evaluation and for the pool
     const double budget = 0.25 * (double)node_cap;
-    double t_norm = 0;
     u32 start = 0;
     while (start < n) {
       u32 end = start;
@@ -1023,7 +1023,6 @@ struct hvm_hip_ctx {
       }
       if (!reset_heap()) return HVM_HIP_E_DEVICE;
       if (!boot_batch(entries + start, end - start)) return HVM_HIP_E_DEVICE;
-      double ta = now_s();
       if (!normalize()) return HVM_HIP_E_EVAL;
       // Per-net readback, as the IO driver does for its one root (src/run.cu:857-925).
       for (u32 j = start; j < end; ++j) {
@@ -1046,7 +1045,6 @@ struct hvm_hip_ctx {
         try { canon = hvm::canonical_tree_text(text); } catch (std::exception& ex) { canon = "<unparsable>"; }
         outs[j].canonical = dup_cstr(canon);
       }
-      t_norm += now_s() - ta;
       start = end;
     }
     if (total) {

@@ -100,3 +100,39 @@ def test_same_result_as_plain_evaluator(hip, cctx):
     cctx.load(w)
     b = cctx.eval()
     assert (a.itrs, a.kinds, a.bytes, a.canonical) == (b.itrs, b.kinds, b.bytes, b.canonical)
+
+
+def stress_code(depth, length):
+    """examples/stress/main.hvm of the reference with its sizes as parameters:
+    2^depth sequential loops of `length` iterations, summed."""
+    return f"""@fun = (?((@fun__C0 @fun__C1) a) a)
+@fun__C0 = a
+  & @loop ~ ({length} a)
+@fun__C1 = ({{a b}} d)
+  &! @fun ~ (a $([+] $(c d)))
+  &! @fun ~ (b c)
+@loop = (?((0 @loop__C0) a) a)
+@loop__C0 = a
+  & @loop ~ a
+@main = a
+  & @fun ~ ({depth} a)
+"""
+
+
+@pytest.mark.parametrize("depth,length", [(4, 300), (10, 64)])
+def test_fast_calls_on_loops(hip, orc, depth, length):
+    # Every loop iteration is a CALL whose caller passes (n, k): the fast
+    # CALL performs the ANNI, the SWITCH and the branch ANNI in the CALL.
+    # Same counts of every kind and the same bytes as the oracle's.
+    w = hip.compile_book(stress_code(depth, length))
+    o = orc.run_oracle(w)
+    with hip.Context(time_limit_s=120) as plain:
+        plain.load(w)
+        a = plain.eval()
+    with hip.Context(time_limit_s=120, flags=hip.COMPILED_CALLS) as comp:
+        comp.load(w)
+        b = comp.eval()
+    assert a.kinds == b.kinds == o.kinds
+    assert a.bytes == b.bytes == o.bytes
+    assert a.canonical == b.canonical == hip.canonical(o.result)
+    assert b.diag["steps"] < a.diag["steps"]
