@@ -61,7 +61,7 @@ def batched(seed: int = 0, n: int = 256) -> Workload:
     return Workload(f"batch(seed={seed}, n={n})", words, lambda ctx: ctx.eval_batch(entries, est)[1])
 
 
-def measure(w: Workload, cfg: dict, repeats: int, time_limit_s: int = 300):
+def measure(w: Workload, cfg: dict, repeats: int, time_limit_s: int = 60):
     """Median eval time (ms) of `repeats` runs after one warm-up; also the
     identity (ITRS, canonical NF) of the result."""
     flags = cfg.get("flags", 0) | (H.NO_READBACK if w.name.startswith("batch") else 0)

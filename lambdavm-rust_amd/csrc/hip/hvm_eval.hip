@@ -460,8 +460,9 @@ __device__ __forceinline__ void put_link(u32* pa, u32* pb, u32& np, u32 p1, u32 
 #define HVM_EVAC_CAP (1u << 16)
 
 // FAST (compiled CALLs): a CALL whose caller's port is a CON node takes that
-// node with the def record and performs the ANNI between it and the def's
-// root node (a CON) instead of creating the root; if the root's first port is
+// node (with the other late takes, once the def record shows a CON root) and
+// performs the ANNI between it and the def's root node instead of creating
+// the root; if the root's first port is
 // a SWI node whose first port is a CON node and the caller passes a number,
 // the SWITCH and the ANNI of the branch pair happen too, without creating
 // either node (src/cmp.rs:106-170 fast SWITCH, 230-262 fast ANNI).  Each is
@@ -664,15 +665,13 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     u32 off_n = 0, off_v = 0;
     bool run = false;
     u64 NA = 0, NB = 0;
-    // fast CALL: the caller's CON node is taken in the def record's round trip
-    const bool fastb = FAST && EARLY && has && rule == HVM_CALL && hvm_get_tag(b) == HVM_CON;
     if (EARLY) {
       u32 tot;
       const u32 o = wave_scan(rule != HVM_CALL ? (nn | (nv << 16)) : 0u, &tot);
       off_n = o & 0xFFFFu; off_v = o >> 16;
       run = has && rule != HVM_CALL && off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
       NA = (run && take_a0) ? node_take(A, hvm_get_val(a)) : 0ull;
-      NB = ((run && take_b0) || fastb) ? node_take(A, hvm_get_val(b)) : 0ull;
+      NB = (run && take_b0) ? node_take(A, hvm_get_val(b)) : 0ull;
     }
     // A CALL's def record with its first template pairs (one 128-byte load).
     bool copy = false;
@@ -733,7 +732,6 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
             off_xn + xn <= HVM_TBL_X && off_xv + xv <= HVM_TBL_X;
     }
     if (has && !run) defers++;
-    if (fastb && !run && NB) node_put(A, hvm_get_val(b), NB);   // waits: the node goes back
     if (LOC && !evac && !__ballot(run)) {
       // the local net is too full for any redex here: drain it
       evac = true;
@@ -752,11 +750,16 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     // ---- P3: takes of the running lanes that have not taken yet (all of
     //      them in the local-net evaluator; a safe def's copy rule) ----
     const bool take_a = run && take_a0;
+    // fast CALL: a def whose root is a CON (template node 0) against the
+    // caller's CON node takes that node (it is consumed: no put-back, so its
+    // slot is never seen free while the node lives)
+    const bool fastb = FAST && EARLY && run && rule == HVM_CALL && !copy && hvm_get_tag(b) == HVM_CON &&
+                       d.root == hvm_new_port(HVM_CON, 0);
     const bool take_b = run && (take_b0 || (rule == HVM_CALL && copy) || fastb);
     if (!EARLY) {
       NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
       NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
-    } else if (run && rule == HVM_CALL && copy) {
+    } else if (run && rule == HVM_CALL && (copy || fastb)) {
       NB = node_take(A, hvm_get_val(b));
     }
     const u64* tpre = dx.node;
@@ -833,8 +836,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     u32 skip_m = 0;
     const u64 R0 = dx.node[0], S0 = dx.node[1], C0 = dx.node[2];
     if (FAST && run && fastb) {
-      fa = d.root == hvm_new_port(HVM_CON, 0);
-      fs = fa && hvm_get_fst(R0) == hvm_new_port(HVM_SWI, 1) && hvm_get_fst(S0) == hvm_new_port(HVM_CON, 2) &&
+      fa = true;
+      fs = hvm_get_fst(R0) == hvm_new_port(HVM_SWI, 1) && hvm_get_fst(S0) == hvm_new_port(HVM_CON, 2) &&
            hvm_get_tag(B1) == HVM_NUM;
       skip_m = fa ? (fs ? 7u : 1u) : 0u;
     }
@@ -886,20 +889,16 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         }
         for (u32 i = HVM_PRE_NODE; i < d.nlen; ++i)
           net_put<LOC>(A, N, tn[i], adjust_pair(A.pairs[d.node_off + i], tn, tv));
-        if (FAST && fastb) {
-          if (!fa) {
-            net_put<LOC>(A, N, hvm_get_val(b), NB);   // not a fast CALL: the caller's node goes back
-          } else {
-            // the slots of the nodes not created go back; s0's holds the
-            // SWITCH's (n-1, continuation) node when n > 0
-            const bool nz = fs && hvm_get_u24(hvm_get_val(B1)) != 0;
-            node_free<LOC>(A, N, tn[r0]);
-            if (fs) node_free<LOC>(A, N, tn[c0]);
-            if (fs && !nz) node_free<LOC>(A, N, tn[s0]);
-            if (nz)
-              net_put<LOC>(A, N, tn[s0], hvm_new_pair(hvm_new_port(HVM_NUM, hvm_new_u24(hvm_get_u24(hvm_get_val(B1)) - 1)),
-                                                      adjust_port(hvm_get_snd(S0), tn, tv)));
-          }
+        if (FAST && fa) {
+          // the slots of the nodes not created go back; s0's holds the
+          // SWITCH's (n-1, continuation) node when n > 0
+          const bool nz = fs && hvm_get_u24(hvm_get_val(B1)) != 0;
+          node_free<LOC>(A, N, tn[r0]);
+          if (fs) node_free<LOC>(A, N, tn[c0]);
+          if (fs && !nz) node_free<LOC>(A, N, tn[s0]);
+          if (nz)
+            net_put<LOC>(A, N, tn[s0], hvm_new_pair(hvm_new_port(HVM_NUM, hvm_new_u24(hvm_get_u24(hvm_get_val(B1)) - 1)),
+                                                    adjust_port(hvm_get_snd(S0), tn, tv)));
         }
       } else if (rule == HVM_COMM) {
         var_put<LOC>(A, N, tv[0], HVM_NONE);
