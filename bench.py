@@ -11,8 +11,8 @@ library.  Parity (ITRS and normal form vs the reference run-c golden record)
 is checked on a warmup step outside the timed region.  The timed steps
 interpret the book as run-cu does; the compiled-CALL mode (def templates
 pre-reduced at load, same ITRS) and the 64-bit-port evaluator are checked
-and timed after them and reported in "mode_compare" (`--compiled` swaps the
-first two).
+and timed after them and reported in "mode_compare" (`--mode` picks which
+of the three the timed steps use).
 
 Prints one JSON line (see the contract in the repository instructions).
 """
@@ -121,10 +121,10 @@ def main():
                     help="turn-length scheduler of the timed steps (the other one is timed after them)")
     ap.add_argument("--no-sched-compare", action="store_true",
                     help="do not time the other scheduler (profiling runs: only the bench's own evaluations)")
-    ap.add_argument("--compiled", action="store_true",
-                    help="time the evaluator with compiled CALLs (templates pre-reduced at load); by "
-                         "default the timed steps interpret the book as run-cu does and the compiled "
-                         "variant is timed after them")
+    ap.add_argument("--mode", choices=("interpreted", "compiled", "wide"), default="interpreted",
+                    help="evaluator of the timed steps: the book interpreted as run-cu does (default), "
+                         "compiled CALLs (templates pre-reduced at load, fast CALL paths), or the "
+                         "64-bit-port evaluator; the other two are checked and timed after them")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (torch), to calibrate "
@@ -154,7 +154,10 @@ def main():
     gcase = next((c for c in golden["workloads"]
                   if c["generator"] == "bitonic" and c["params"] == {"k": args.k, "seed": 0}), None)
 
-    ctx = hvm_hip.Context(time_limit_s=600)
+    MODES = {"interpreted": 0, "compiled": hvm_hip.COMPILED_CALLS, "wide": hvm_hip.WIDE}
+    mode = MODES[args.mode]
+    wide = args.mode == "wide"
+    ctx = hvm_hip.Context(time_limit_s=600, flags=mode)
     ctx.load(words)
     # parity step (untimed): full readback, compared with the golden record
     r0 = ctx.eval()
@@ -178,10 +181,10 @@ def main():
         else:
             ctx.set_scheduler(hvm_hip.SCHED_PLAIN)
 
-    mode = hvm_hip.COMPILED_CALLS if args.compiled else 0
     ctx = hvm_hip.Context(time_limit_s=600, flags=hvm_hip.NO_READBACK | mode)
     ctx.load(words)
-    use(ctx, args.sched)
+    if not wide:
+        use(ctx, args.sched)
     for _ in range(max(0, args.warmup - 1)):
         ctx.eval()
     runs = []
@@ -192,7 +195,7 @@ def main():
     # the other scheduler on the same workload, after the timed region
     other = "plain" if args.sched == "learned" else "learned"
     compare = {args.sched + "_mips": None, other + "_mips": None}
-    if weights is not None and not args.no_sched_compare:
+    if weights is not None and not args.no_sched_compare and not wide:
         use(ctx, other)
         ctx.eval()
         t2 = time.perf_counter()
@@ -207,10 +210,9 @@ def main():
     # golden record, then the same step count
     variant = []
     if not args.no_sched_compare:
-        modes = [("interpreted" if args.compiled else "compiled_calls",
-                  0 if args.compiled else hvm_hip.COMPILED_CALLS),
-                 ("wide_ports", hvm_hip.WIDE)]
-        for name, vflag in modes:
+        for name, vflag in MODES.items():
+            if name == args.mode:
+                continue
             with hvm_hip.Context(time_limit_s=600, flags=vflag) as vctx:
                 vctx.load(words)
                 rv = vctx.eval()
@@ -239,11 +241,17 @@ def main():
     # already normal, as rocprofv3 lists them
     launches = sum(r.launches for r in runs)
     launch_ms = sum(r.launch_ms for r in runs)
+    if wide:
+        # the wide turn loop is timed by the wall clock (its kernels carry no events)
+        kernel_ms = launch_ms = sum(r.eval_ms for r in runs)
     bytes_total = sum(r.bytes for r in runs)
     avg_launch_ms = launch_ms / launches
     bytes_per_launch = bytes_total / launches
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(args.k)
+    # the counter profile is of the interpreting kernel
+    traffic, traffic_src = measured_traffic(args.k) if args.mode == "interpreted" else (None, None)
+    kernel_name = {"interpreted": "hvm_turn_kernel", "compiled": "hvm_turn_fast_kernel",
+                   "wide": "hvm_wide_turn_kernel"}[args.mode]
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -259,11 +267,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"bitonic sort of 2^{args.k} u24 keys (SplitMix64 seed 0), depth-{args.k} key tree",
                    "keys": 1 << args.k, "itrs_per_step": itrs, "scheduler": args.sched,
-                   "mode": "compiled_calls" if args.compiled else "interpreted"},
+                   "mode": args.mode},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "hvm_turn_kernel", "launches_per_step": launches / args.steps,
+                     "kernel": kernel_name, "launches_per_step": launches / args.steps,
                      "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
                      "avg_launch_ms": round(avg_launch_ms, 4)},
         "kernel_mips": round(itrs * args.steps / (kernel_ms * 1e-3) / 1e6, 2),
@@ -273,7 +281,8 @@ def main():
         "mode_compare": variant,
     }
     compare[args.sched + "_mips"] = out["value"]
-    out["atomic_roofline"] = atomic_roofline(args.k, itrs * args.steps / (kernel_ms * 1e-3))
+    if args.mode == "interpreted":
+        out["atomic_roofline"] = atomic_roofline(args.k, itrs * args.steps / (kernel_ms * 1e-3))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(words, itrs)
     print(json.dumps(out), flush=True)
