@@ -10,7 +10,8 @@ FLUSH, SEEK on a bad descriptor, an unknown IO function).  Each runs in a
 scratch directory holding the files of FILES, with STDIN on standard input,
 as the reference's tests/run.rs:67-110 runs them.  Recorded per program: the
 program text, the output lines other than ITRS/TIME/MIPS/LEAK, the result
-with canonical var names, and ITRS.
+with canonical var names, ITRS, and the reference's own snapshot of the
+program's output where it has one.
 """
 import glob
 import json
@@ -29,11 +30,28 @@ import oracle  # noqa: E402
 
 REF = "/root/reference"
 STDIN = "io from the tests\n"
-# Files the programs open, by name, with contents of this repository's own.
+# Files the programs open, by name.  LICENSE starts as the Apache License 2.0
+# text does (the file the reference's IO tests read from its repository root),
+# so that run-c's output can also be checked against the reference's own
+# snapshots of these programs (tests/snapshots/run__io_file@*.snap).
 FILES = {
-    "LICENSE": "HVM IO fixture: these forty-seven bytes are read by the IO tests.\nsecond line\n",
+    "LICENSE": "                                 Apache License\n                           Version 2.0, January 2004\n",
     "README.md": "fixture\n",
 }
+
+
+def reference_snapshot(name):
+    """The normalised output the reference's tests/run.rs recorded for an IO
+    program (lines after the snapshot header; None if there is none)."""
+    path = os.path.join(REF, "tests/snapshots", "run__io_file@" + name.replace("tests/programs/", "").replace("examples/", "")
+                        .replace("/", "__") + ".snap")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read().split("---\n", 2)[2]
+    lines = text.rstrip("\n").split("\n")
+    result = [l[len("Result: "):] for l in lines if l.startswith("Result: ")]
+    return {"lines": [l for l in lines if not l.startswith("Result: ")],
+            "result": H.canonical(result[0]) if result else None}
 
 
 def run_reference(code):
@@ -66,6 +84,9 @@ def main():
         code = open(path).read()
         name = os.path.relpath(path, REF) if path.startswith(REF) else os.path.relpath(path, ROOT)
         rec = dict(name=name, code=code, **run_reference(code))
+        snap = reference_snapshot(name) if path.startswith(REF) else None
+        if snap is not None:
+            rec["reference_snapshot"] = snap
         print(name, rec["result"], rec["itrs"], rec["lines"])
         out["programs"].append(rec)
     with open(os.path.join(HERE, "io.json"), "w") as f:

@@ -136,6 +136,21 @@ def test_io_fixture_programs_compile(hip):
     # The program of this repository is the one the fixture was made from.
     with open(os.path.join(ROOT, "tests", "programs", "io", "echo_seek.hvm")) as f:
         assert f.read() == next(c for c in fx["programs"] if c["name"].endswith("echo_seek.hvm"))["code"]
-    # Its stdin READ and the 5 bytes read at offset 10 of the LICENSE fixture.
+    # Its stdin READ and the 5 bytes read at offset 33 of the LICENSE fixture.
     echo = next(c for c in fx["programs"] if c["name"].endswith("echo_seek.hvm"))
-    assert echo["lines"] == [fx["stdin"][:12] + fx["files"]["LICENSE"][10:15]]
+    assert echo["lines"] == [fx["stdin"][:12] + fx["files"]["LICENSE"][33:38]]
+
+
+def test_io_fixture_matches_reference_snapshots():
+    """run-c's output on the reference's IO programs (tests/golden/io.json)
+    equals the reference's own snapshots of them (tests/run.rs:67-110 runs
+    them from the repository root, whose LICENSE the fixture starts like)."""
+    import json
+    import os
+    with open(os.path.join(ROOT, "tests", "golden", "io.json")) as f:
+        fx = json.load(f)
+    snaps = [c for c in fx["programs"] if "reference_snapshot" in c]
+    assert len(snaps) == 6
+    for c in snaps:
+        assert c["lines"] == c["reference_snapshot"]["lines"], c["name"]
+        assert c["result"] == c["reference_snapshot"]["result"], c["name"]
