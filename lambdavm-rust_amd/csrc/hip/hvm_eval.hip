@@ -1319,12 +1319,13 @@ __device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__u
 
 __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u32 clock_khz) {
   __shared__ float f[HVM_FEAT], h1[HVM_HID], h2[HVM_HID], y[HVM_ACTIONS];
-  __shared__ u32 s_go, s_n;
+  __shared__ u32 s_go, s_n, s_steady;
   HvmCtl* c = A.ctl;
   HvmSched* s = A.sched;
   const u32 t = threadIdx.x;
   if (t == 0) {
     s_go = 0;
+    s_steady = 0;
     if (!c->done) {
       u32 n = c->n_out;
       c->turns += 1;
@@ -1347,6 +1348,9 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
           rec[HVM_FEAT + 3] = dt_us;
         }
         sched_features(c, s, n, A.nwaves, dt_us, f);
+        // few redexes, not growing: independent sequential chains, which a
+        // redistribution cannot spread; the next turn runs without idle quit
+        s_steady = n <= A.nwaves / 4 && 2 * n <= 3 * c->n_in;
         c->n_in = n;
         c->n_out = 0;
         c->idle_waves = 0;
@@ -1402,6 +1406,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
   if (t == 0) {
     c->max_steps = c->fixed_steps ? c->fixed_steps : kTurnLen[action];
     c->idle_quit = kIdlePct[action] ? max(1u, (A.nwaves * kIdlePct[action]) / 100u) : A.idle_quit;
+    if (s_steady) c->idle_quit = 0;
     s->last_action = action;
     if (s->trace && s->trace_len < s->trace_cap) {
       float* rec = s->trace + (size_t)s->trace_len * HVM_TRACE_REC;

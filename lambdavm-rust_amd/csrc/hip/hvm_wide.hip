@@ -473,7 +473,11 @@ __global__ void hvm_wide_between_kernel(HvmWideArgs A) {
   if (C->done) return;
   if (A.node[2ull * A.node_cap] || A.node[2ull * A.node_cap + 1] || A.vars[A.vars_cap])
     C->err |= HVM_ERR_BAD_PORT;
-  C->n_in = min(C->n_out, A.bag_cap);
+  const u32 n = min(C->n_out, A.bag_cap);
+  // few redexes, not growing: sequential chains that a redistribution cannot
+  // spread; the next turn runs without idle quit
+  C->idle_quit = (n <= A.nwaves / 4 && 2 * n <= 3 * C->n_in) ? 0u : C->idle_base;
+  C->n_in = n;
   C->n_out = 0;
   C->in_head = 0;
   C->parity ^= 1u;
@@ -553,6 +557,7 @@ bool HvmWide::boot(unsigned long long x, unsigned long long y) {
   c.n_in = 1;
   c.max_steps = turn_steps ? std::min<u32>(turn_steps, 1u << 20) : 4096u;
   c.idle_quit = idle_pct > 100 ? 0u : std::max<u32>(1u, (u32)(((u64)a.nwaves * idle_pct) / 100));
+  c.idle_base = c.idle_quit;
   return ok(hipMemcpyAsync(a.vars, &root_none, 8, hipMemcpyHostToDevice, stream), "boot") &&
          ok(hipMemcpyAsync(a.bag[0], redex, 16, hipMemcpyHostToDevice, stream), "boot") &&
          ok(hipMemcpyAsync(a.ctl, &c, sizeof c, hipMemcpyHostToDevice, stream), "boot") &&

@@ -26,7 +26,7 @@
 struct HvmWideCtl {
   uint32_t n_in, n_out, in_head, err;
   uint32_t parity, max_steps, turns, done;
-  uint32_t idle_waves, idle_quit, pad[6];
+  uint32_t idle_waves, idle_quit, idle_base, pad[5];
 };
 
 struct HvmWideArgs {
