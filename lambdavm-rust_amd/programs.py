@@ -112,6 +112,27 @@ def sum_tree(depth: int = 20) -> Tuple[str, int]:
     return code, (1 << depth) % U24
 
 
+def stress(depth: int = 8, length: int = 65536) -> Tuple[str, int]:
+    """The reference's stress program (examples/stress/main.hvm: 2^depth
+    independent loops of `length` iterations each, their results summed)
+    with its two sizes as parameters; the shipped example is (8, 65536).
+    Its normal form is 0."""
+    code = f"""
+@fun = (?((@fun__C0 @fun__C1) a) a)
+@fun__C0 = a
+  & @loop ~ ({length} a)
+@fun__C1 = ({{a b}} d)
+  &! @fun ~ (a $([+] $(c d)))
+  &! @fun ~ (b c)
+@loop = (?((0 @loop__C0) a) a)
+@loop__C0 = a
+  & @loop ~ a
+@main = a
+  & @fun ~ ({depth} a)
+"""
+    return code, 0
+
+
 def tree_gen(depth: int = 30) -> Tuple[str, str]:
     """A full binary tree of CON nodes with 2^depth leaves `1`, built
     recursively: its normal form alone holds 2^depth - 1 nodes (depth 30 does
