@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -79,3 +80,19 @@ def test_no_device_is_an_error_not_a_fallback(hip):
     assert b"device" in L.hvm_hip_last_error(None).lower()
     with pytest.raises(hip.HvmError):
         hip.run_code("@main = 1")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # One GPU per box: --gpus above 1 exits with a message before any GPU work.
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr
+
+
+def test_wide_mode_is_an_error_without_a_device(hip):
+    has_gpu = os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK)
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    with pytest.raises(hip.HvmError):
+        hip.Context(flags=hip.WIDE)
