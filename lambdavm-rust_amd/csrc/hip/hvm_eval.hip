@@ -78,6 +78,7 @@ struct Lane {
   u64 spec;                     // static CALLs (low half) and ANNIs (high half)
                                 // of the compiled defs this lane expanded
   u32 fsw0, fswn;               // SWITCHes performed inside fast CALLs
+  u32 voids;                    // VOID redexes counted where they were created
 };
 
 // Wave-wide interaction counts (wave-uniform, kept in scalar registers).
@@ -311,8 +312,15 @@ __device__ __forceinline__ void link_batch(WaveLds& L, LocalNet* N, Lane& ln, co
   }
   if (__builtin_expect(busy, 0)) atomicOr(&A.ctl->err, (u32)HVM_ERR_LINK_LOOP);
 #pragma unroll
-  for (int i = 0; i < HVM_LB; ++i)
-    if (got & (1u << i)) push_redex<LOC>(L, N, ln, A, hvm_new_pair(pa[i], pb[i]));
+  for (int i = 0; i < HVM_LB; ++i) {
+    if (!(got & (1u << i))) continue;
+    // Two nullary ports (REF, ERA, NUM) make a VOID redex: the rule does
+    // nothing (src/hvm.c:909-911), so it is counted here, where it is
+    // created, instead of taking a lane-step later.  ITRS and the kind
+    // counts are those of running it.
+    if (hvm_get_tag(pa[i]) - 1u < 3u && hvm_get_tag(pb[i]) - 1u < 3u) ln.voids++;
+    else push_redex<LOC>(L, N, ln, A, hvm_new_pair(pa[i], pb[i]));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -492,6 +500,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   ln.spills = 0;
   ln.spec = 0;
   ln.fsw0 = ln.fswn = 0;
+  ln.voids = 0;
   u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0;
 
   if (LOC) {
@@ -1154,6 +1163,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     // per lane each half stays below 2^32 for turns of up to 2^20 steps
     kinds[HVM_K_CALL] += wave_sum64(ln.spec & 0xFFFFFFFFull);
     kinds[HVM_K_ANNI] += wave_sum64(ln.spec >> 32);
+    kinds[HVM_K_VOID] += wave_sum64(ln.voids);
     if (FAST) {
       kinds[HVM_K_SWIT0] += wave_sum64(ln.fsw0);
       kinds[HVM_K_SWITN] += wave_sum64(ln.fswn);

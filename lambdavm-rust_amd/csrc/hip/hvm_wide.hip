@@ -55,7 +55,7 @@ struct WLds {
 struct WLane {
   u32 bot, top;
   u64 bytes, spec;
-  u32 spills;
+  u32 spills, voids;
 };
 
 // ---- net accessors (every shared access is a device-scope atomic) ----------
@@ -121,7 +121,12 @@ __device__ u64 enter(const HvmWideArgs& A, u64 b) {
 __device__ void link(WLds& L, WLane& ln, const HvmWideArgs& A, u64 a, u64 b) {
   for (u32 it = 0; it < W_LINK_CAP; ++it) {
     if (wtag(a) != HVM_VAR && wtag(b) == HVM_VAR) { u64 t = a; a = b; b = t; }
-    if (wtag(a) != HVM_VAR) { push(L, ln, A, a, b); return; }
+    if (wtag(a) != HVM_VAR) {
+      // two nullary ports make a VOID redex, counted here instead of run
+      if (wtag(a) - 1u < 3u && wtag(b) - 1u < 3u) ln.voids++;
+      else push(L, ln, A, a, b);
+      return;
+    }
     b = enter(A, b);
     u64* slot = &A.vars[vslot(A, wval(a))];
     u64 a2 = xchg(slot, b);
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
   const u32 n_lo = 1 + w * A.node_slice, v_lo = 1 + w * A.vars_slice;
   if (ws.node_cur >= A.node_slice) ws.node_cur = 0;
   if (ws.vars_cur >= A.vars_slice) ws.vars_cur = 0;
-  WLane ln{0, 0, 0, 0, 0};
+  WLane ln{0, 0, 0, 0, 0, 0};
   u64 kinds[HVM_K_KINDS];
 #pragma unroll
   for (int k = 0; k < HVM_K_KINDS; ++k) kinds[k] = 0;
@@ -448,6 +453,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
 
   kinds[HVM_K_CALL] += wave_sum64(ln.spec & 0xFFFFFFFFull);
   kinds[HVM_K_ANNI] += wave_sum64(ln.spec >> 32);
+  kinds[HVM_K_VOID] += wave_sum64(ln.voids);
   u64 s_itrs = 0;
 #pragma unroll
   for (int k = 0; k < HVM_K_KINDS; ++k) s_itrs += kinds[k];
