@@ -840,7 +840,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     // caller's CON; with a number there and the root's first port a SWI
     // (node 1) whose first port is a CON (node 2) — the pre-order numbering
     // of a def that switches on its first argument — the SWITCH too.
-    bool fa = false, fs = false;
+    bool fa = false, fs = false, shared_k = false;
     const u32 r0 = 0, s0 = 1, c0 = 2;
     u32 skip_m = 0;
     const u64 R0 = dx.node[0], S0 = dx.node[1], C0 = dx.node[2];
@@ -848,6 +848,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       fa = true;
       fs = hvm_get_fst(R0) == hvm_new_port(HVM_SWI, 1) && hvm_get_fst(S0) == hvm_new_port(HVM_CON, 2) &&
            hvm_get_tag(B1) == HVM_NUM;
+      shared_k = fs && hvm_get_tag(hvm_get_snd(S0)) == HVM_VAR && hvm_get_snd(S0) == hvm_get_snd(R0);
       skip_m = fa ? (fs ? 7u : 1u) : 0u;
     }
     bool oper1 = false;
@@ -905,9 +906,13 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
           node_free<LOC>(A, N, tn[r0]);
           if (fs) node_free<LOC>(A, N, tn[c0]);
           if (fs && !nz) node_free<LOC>(A, N, tn[s0]);
+          // the continuation: with one var shared by the SWI and the root
+          // (its only two occurrences) the caller's port replaces it, and
+          // the var goes back unused
+          if (fs && shared_k) var_free<LOC>(A, N, tv[hvm_get_val(hvm_get_snd(S0))]);
           if (nz)
             net_put<LOC>(A, N, tn[s0], hvm_new_pair(hvm_new_port(HVM_NUM, hvm_new_u24(hvm_get_u24(hvm_get_val(B1)) - 1)),
-                                                    adjust_port(hvm_get_snd(S0), tn, tv)));
+                                                    shared_k ? B2 : adjust_port(hvm_get_snd(S0), tn, tv)));
         }
       } else if (rule == HVM_COMM) {
         var_put<LOC>(A, N, tv[0], HVM_NONE);
@@ -990,14 +995,15 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
               // and erases the other; n > 0 erases the zero branch and
               // passes (n-1, continuation) to the other
               const u32 z = adjust_port(hvm_get_fst(C0), tn, tv), sc = adjust_port(hvm_get_snd(C0), tn, tv);
+              const u32 k = shared_k ? B2 : adjust_port(hvm_get_snd(S0), tn, tv);
               if (hvm_get_u24(hvm_get_val(B1)) == 0) {
-                emit(z, adjust_port(hvm_get_snd(S0), tn, tv));
+                emit(z, k);
                 emit(sc, hvm_new_port(HVM_ERA, 0));
               } else {
                 emit(z, hvm_new_port(HVM_ERA, 0));
                 emit(sc, hvm_new_port(HVM_CON, tn[s0]));
               }
-              emit(adjust_port(hvm_get_snd(R0), tn, tv), B2);
+              if (!shared_k) emit(adjust_port(hvm_get_snd(R0), tn, tv), B2);
             } else if (FAST && fa) {
               emit(adjust_port(hvm_get_fst(R0), tn, tv), B1);
               emit(adjust_port(hvm_get_snd(R0), tn, tv), B2);
