@@ -82,6 +82,10 @@ typedef struct hvm_hip_cfg {
  * with its own kernel (one redex per lane per step); no IO driver, no batched
  * nets, no local nets, plain scheduling only. */
 #define HVM_HIP_WIDE 32u
+/* Launch each chunk of turns (turn and between kernels with their timing
+ * events) as a HIP graph captured once per chunk length and replayed, instead
+ * of one launch call per kernel.  The 32-bit evaluators only. */
+#define HVM_HIP_GRAPHS 64u
 #define HVM_HIP_SPEC_NODES 64u
 
 /* Interaction kinds, in the order of hvm_hip_out.kind[]. */

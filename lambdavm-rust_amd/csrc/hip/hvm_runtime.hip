@@ -184,6 +184,7 @@ struct hvm_hip_ctx {
     if (d_pairs) hipFree(d_pairs);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
+    drop_graphs();
     for (hipEvent_t e : tev) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
   }
@@ -375,9 +376,58 @@ struct hvm_hip_ctx {
 
   HvmCtl host_ctl{};
   std::vector<hipEvent_t> tev;  // a start/stop event pair per turn of a chunk
+  // HVM_HIP_GRAPHS: one instantiated graph per chunk length (2, 4, ..., 64)
+  // holding the chunk's (event, turn, event, between) launches; rebuilt when
+  // the turn arguments change
+  hipGraphExec_t chunk_graph[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  HvmTurnArgs graph_args{};
+
+  void drop_graphs() {
+    for (hipGraphExec_t& g : chunk_graph) {
+      if (g) hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
+  // Issues one chunk of `chunk` turns: captured once into a graph and
+  // replayed, or launched directly.
+  bool launch_chunk(const HvmTurnArgs& a, u32 chunk) {
+    const bool graphs = (cfg.flags & HVM_HIP_GRAPHS) != 0;
+    auto direct = [&](hipStream_t st) {
+      for (u32 i = 0; i < chunk; ++i) {
+        if (!graphs) hipEventRecord(tev[2 * i], st);
+        if (!ck(hvm_launch_turn(&a, st), "launch turn")) return false;
+        if (!graphs) hipEventRecord(tev[2 * i + 1], st);
+        if (!ck(hvm_launch_between(&a, clock_khz, st), "launch between")) return false;
+      }
+      return true;
+    };
+    if (!graphs) return direct(stream);
+    if (memcmp(&graph_args, &a, sizeof a) != 0) {
+      drop_graphs();
+      graph_args = a;
+    }
+    u32 k = 0;
+    while ((2u << k) < chunk) ++k;
+    if (!chunk_graph[k]) {
+      hipGraph_t g = nullptr;
+      if (!ck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
+      bool ok = direct(stream);
+      hipError_t e = hipStreamEndCapture(stream, &g);
+      if (!ok) { if (g) hipGraphDestroy(g); return false; }
+      if (!ck(e, "end capture")) return false;
+      e = hipGraphInstantiate(&chunk_graph[k], g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (!ck(e, "instantiate graph")) return false;
+    }
+    // the graph holds kernels only; the chunk is timed as a whole
+    hipEventRecord(tev[0], stream);
+    if (!ck(hipGraphLaunch(chunk_graph[k], stream), "launch graph")) return false;
+    return ck(hipEventRecord(tev[1], stream), "record chunk end");
+  }
 
   HvmTurnArgs turn_args() const {
     HvmTurnArgs a;
+    memset(&a, 0, sizeof a);   // padding too: launch_chunk compares argument blocks
     a.node = node; a.vars = vars;
     a.node_cap = node_cap; a.vars_cap = vars_cap;
     a.node_slice = node_slice; a.vars_slice = vars_slice;
@@ -424,20 +474,23 @@ struct hvm_hip_ctx {
     u32 turns_before = host_ctl.turns;
     for (;;) {
       if (deadline > 0 && now_s() > deadline) return fail("time limit exceeded");
-      for (u32 i = 0; i < chunk; ++i) {
-        hipEventRecord(tev[2 * i], stream);
-        if (!ck(hvm_launch_turn(&a, stream), "launch turn")) return false;
-        hipEventRecord(tev[2 * i + 1], stream);
-        if (!ck(hvm_launch_between(&a, clock_khz, stream), "launch between")) return false;
-      }
+      if (!launch_chunk(a, chunk)) return false;
       if (!ck(hipMemcpyAsync(&host_ctl, ctl, sizeof(HvmCtl), hipMemcpyDeviceToHost, stream), "ctl readback")) return false;
       if (!ck(hipStreamSynchronize(stream), "turn sync")) return false;
       u32 ran = host_ctl.turns - turns_before;
-      for (u32 i = 0; i < chunk; ++i) {
+      if (cfg.flags & HVM_HIP_GRAPHS) {
+        // one span for the chunk (its between kernels included)
         float ms = 0;
-        hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]);
-        if (i < ran) kernel_ms += ms;
+        hipEventElapsedTime(&ms, tev[0], tev[1]);
+        kernel_ms += ms * (float)ran / (float)chunk;
         launch_ms += ms;
+      } else {
+        for (u32 i = 0; i < chunk; ++i) {
+          float ms = 0;
+          hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]);
+          if (i < ran) kernel_ms += ms;
+          launch_ms += ms;
+        }
       }
       launches += chunk;
       turns += ran;

@@ -30,6 +30,7 @@ PROFILE = 4
 LOCAL_NET = 8
 COMPILED_CALLS = 16
 WIDE = 32
+GRAPHS = 64
 SPEC_NODES = 64
 
 # Scheduler (include/hvm_hip.h)

@@ -156,3 +156,24 @@ def test_context_recovers_after_an_error(golden, hip):
         ctx.load(hip.compile_book(P.sum_tree(4)[0]))
         r = ctx.eval()
         assert r.err == 0 and r.itrs == case["itrs"] and r.result == "16"
+
+
+def test_graph_replay_matches_direct_launches(golden, hip):
+    # HVM_HIP_GRAPHS: chunks of turns replayed from captured graphs, many
+    # replays per evaluation, several evaluations (with eager host work and
+    # synchronisations between them) on one context; every one identical to
+    # the run-c record and to direct launches.
+    case = next(c for c in golden["workloads"] if c["name"] == "bitonic_k8_seed3")
+    code, expect = P.bitonic(8, 3)
+    w = hip.compile_book(code)
+    for flags in (hip.GRAPHS, hip.GRAPHS | hip.COMPILED_CALLS):
+        with hip.Context(time_limit_s=120, flags=flags, turn_steps=7) as ctx:
+            ctx.load(w)
+            for _ in range(3):
+                r = ctx.eval()
+                assert r.itrs == case["itrs"]
+                assert P.flatten_tree(P.parse_tree_text(r.result)) == expect
+    full = next(c for c in golden["workloads"] if c.get("full_size") and c["generator"] == "radix")
+    code, expect = P.radix(**full["params"])
+    r = hip.run_code(code, time_limit_s=120, flags=hip.GRAPHS)
+    assert r.itrs == full["itrs"] and r.result == str(expect)
