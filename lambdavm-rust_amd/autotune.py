@@ -13,7 +13,9 @@ hvm_hip_cfg):
   * turn_steps   — a fixed share interval (steps per turn) instead of the
                    scheduler's choice;
   * flags        — HVM_HIP_LOCAL_NET (per-wave LDS nets) and/or
-                   HVM_HIP_COMPILED_CALLS (def templates pre-reduced at load).
+                   HVM_HIP_COMPILED_CALLS (def templates pre-reduced at load);
+  * graphs       — HVM_HIP_GRAPHS (turn chunks replayed from HIP graphs) or
+                   not, or-ed into the flags.
 The per-lane redex deque (HVM_DEQ) and the LDS net sizes are compile-time
 constants of the kernels and are not searched here.
 
@@ -64,8 +66,9 @@ def batched(seed: int = 0, n: int = 256) -> Workload:
 def measure(w: Workload, cfg: dict, repeats: int, time_limit_s: int = 60):
     """Median eval time (ms) of `repeats` runs after one warm-up; also the
     identity (ITRS, canonical NF) of the result."""
-    flags = cfg.get("flags", 0) | (H.NO_READBACK if w.name.startswith("batch") else 0)
-    with H.Context(time_limit_s=time_limit_s, **dict(cfg, flags=flags)) as ctx:
+    flags = cfg.get("flags", 0) | cfg.get("graphs", 0) | (H.NO_READBACK if w.name.startswith("batch") else 0)
+    ctx_cfg = {k: v for k, v in cfg.items() if k != "graphs"}
+    with H.Context(time_limit_s=time_limit_s, **dict(ctx_cfg, flags=flags)) as ctx:
         ctx.load(w.words)
         first = w.runner(ctx)
         ms = []
@@ -91,6 +94,7 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
         ("idle_pct", [10, 25, 75, 101, 0]),
         ("turn_steps", [96, 512, 4096, 0]),
         ("flags", [H.COMPILED_CALLS, H.LOCAL_NET, H.LOCAL_NET | H.COMPILED_CALLS, 0]),
+        ("graphs", [H.GRAPHS, 0]),
     ]
     best, t_best = dict(base), t_def
     rows = [{"cfg": dict(base), "ms": round(t_def, 3)}]

@@ -99,7 +99,8 @@ def test_batch_matches_per_net_reference(hip, which):
     if which == "tuned":
         if _tuned_batch_cfg() is None:
             pytest.skip("no tuned configuration committed")
-        cfg = _tuned_batch_cfg()
+        cfg = dict(_tuned_batch_cfg())
+        cfg["flags"] = cfg.get("flags", 0) | cfg.pop("graphs", 0)   # the autotuner's graphs knob
     with hip.Context(time_limit_s=300, **cfg) as ctx:
         ctx.load(hip.compile_book(code))
         per, total = ctx.eval_batch(entries, est)
