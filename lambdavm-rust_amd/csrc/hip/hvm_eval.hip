@@ -48,6 +48,13 @@ __device__ __forceinline__ u32 vars_slot(const HvmTurnArgs& A, u32 val) { return
 __device__ __forceinline__ u64 node_take(const HvmTurnArgs& A, u32 loc) {
   return __hip_atomic_exchange(&A.node[node_slot(A, loc)], 0ull, __ATOMIC_RELAXED, AGENT);
 }
+// A take as a load: only the lane holding the redex that points at a node
+// ever reaches it, so reading it and zeroing the slot once the interaction
+// commits (node_put of 0) is a take; nothing is removed before then, so a
+// lane that retries has nothing to put back.
+__device__ __forceinline__ u64 node_peek(const HvmTurnArgs& A, u32 loc) {
+  return __hip_atomic_load(&A.node[node_slot(A, loc)], __ATOMIC_RELAXED, AGENT);
+}
 __device__ __forceinline__ void node_put(const HvmTurnArgs& A, u32 loc, u64 v) {
   __hip_atomic_store(&A.node[node_slot(A, loc)], v, __ATOMIC_RELAXED, AGENT);
 }
@@ -463,6 +470,14 @@ __device__ __forceinline__ void put_link(u32* pa, u32* pb, u32& np, u32 p1, u32 
 #ifndef HVM_EARLY_TAKES
 #define HVM_EARLY_TAKES 1
 #endif
+// HBM evaluator: takes as coherent loads, slots zeroed at commit
+// (node_peek), instead of exchanges.  Parity-equal and within 1% of the
+// exchanges on the sort, radix, sum-tree and Church workloads, interpreted
+// and with compiled CALLs: a device-scope load's round trip costs what an
+// exchange's does here.  Off by default.
+#ifndef HVM_PEEK_TAKES
+#define HVM_PEEK_TAKES 0
+#endif
 
 // Steps a wave may spend draining its local net before giving up.
 #define HVM_EVAC_CAP (1u << 16)
@@ -646,9 +661,12 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     }
     // HBM evaluator: takes are issued before the reservation, so that their
     // round trip is the one of the CALLs' def records and the allocator's
-    // first window; a lane that ends up waiting puts its nodes back.  The
-    // local-net evaluator takes after the reservation (P3).
+    // first window.  With PEEK a take is a load and the slot is zeroed when
+    // the interaction commits, so a lane that retries has removed nothing.
+    // The local-net evaluator takes after the reservation (P3).
     constexpr bool EARLY = !LOC && HVM_EARLY_TAKES;
+    constexpr bool PEEK = EARLY && HVM_PEEK_TAKES;
+    auto hbm_take = [](const HvmTurnArgs& A, u32 loc) { return PEEK ? node_peek(A, loc) : node_take(A, loc); };
     const bool take_a0 = has && (rule == HVM_ANNI || rule == HVM_COMM);
     const bool take_b0 = has && (rule == HVM_ERAS || rule == HVM_ANNI || rule == HVM_COMM ||
                                  rule == HVM_OPER || rule == HVM_SWIT || (LOC && rule == HVM_LEAK));
@@ -679,8 +697,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       const u32 o = wave_scan(rule != HVM_CALL ? (nn | (nv << 16)) : 0u, &tot);
       off_n = o & 0xFFFFu; off_v = o >> 16;
       run = has && rule != HVM_CALL && off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
-      NA = (run && take_a0) ? node_take(A, hvm_get_val(a)) : 0ull;
-      NB = (run && take_b0) ? node_take(A, hvm_get_val(b)) : 0ull;
+      NA = (run && take_a0) ? hbm_take(A, hvm_get_val(a)) : 0ull;
+      NB = (run && take_b0) ? hbm_take(A, hvm_get_val(b)) : 0ull;
     }
     // A CALL's def record with its first template pairs (one 128-byte load).
     bool copy = false;
@@ -769,7 +787,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
       NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
     } else if (run && rule == HVM_CALL && (copy || fastb)) {
-      NB = node_take(A, hvm_get_val(b));
+      NB = hbm_take(A, hvm_get_val(b));
     }
     const u64* tpre = dx.node;
     const u64* rpre = dx.rbag;
@@ -818,8 +836,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       if ((take_a && NA == 0 && !is_loc_val(hvm_get_val(a)) && hvm_get_val(a) >= A.node_cap) ||
           (take_b && NB == 0 && !is_loc_val(hvm_get_val(b)) && hvm_get_val(b) >= A.node_cap))
         atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT);   // not a node of this net
-      if (take_a && NA) net_put<LOC>(A, N, hvm_get_val(a), NA);
-      if (take_b && NB) net_put<LOC>(A, N, hvm_get_val(b), NB);
+      if (!PEEK && take_a && NA) net_put<LOC>(A, N, hvm_get_val(a), NA);
+      if (!PEEK && take_b && NB) net_put<LOC>(A, N, hvm_get_val(b), NB);
       // slots allocated for this interaction go back (they may be pool claims)
       for (u32 i = 0; i < nn; ++i) node_free<LOC>(A, N, tn[i]);
       for (u32 i = 0; i < nv; ++i) var_free<LOC>(A, N, tv[i]);
@@ -828,6 +846,11 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       ln.top++;
       run = false;
       retries++;
+    }
+    // nodes read by a peek are taken now: their slots are zeroed
+    if (PEEK && run) {
+      if (take_a) node_put(A, hvm_get_val(a), 0ull);
+      if (take_b) node_put(A, hvm_get_val(b), 0ull);
     }
     // taken local nodes are dead now: their slots go back on the stack
     if (LOC && run) {
