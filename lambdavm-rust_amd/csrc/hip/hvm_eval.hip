@@ -479,6 +479,13 @@ __device__ __forceinline__ void put_link(u32* pa, u32* pb, u32& np, u32 p1, u32 
 #define HVM_PEEK_TAKES 0
 #endif
 
+// Template pairs loaded together per chunk of a CALL's creation; the
+// allocation tables are read up to HVM_XCHUNK - 1 entries past a def's own.
+#ifndef HVM_XCHUNK
+#define HVM_XCHUNK 8
+#endif
+static_assert(HVM_TBL_SLACK >= HVM_XCHUNK - 1, "table reads stay inside the tables");
+
 // Steps a wave may spend draining its local net before giving up.
 #define HVM_EVAC_CAP (1u << 16)
 
@@ -900,6 +907,23 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         // the table: offsets are at most HVM_TBL_NODE/VARS, the tables have
         // HVM_TBL_SLACK more entries).
         static_assert(HVM_TBL_SLACK >= 8 && HVM_PRE_NODE % 4 == 0, "table reads stay inside the tables");
+        // Template nodes past the prefetched ones come first, eight at a
+        // time: a chunk's pair loads are issued together, before any store
+        // of the creation.  vmcnt counts loads and stores in issue order, so
+        // a wait for a load behind a write-through store also waits for that
+        // store; loading one pair per node between the stores cost a store
+        // round trip per node.
+        for (u32 c = HVM_PRE_NODE; c < d.nlen; c += HVM_XCHUNK) {
+          u64 xp[HVM_XCHUNK];
+          u32 locs[HVM_XCHUNK];
+#pragma unroll
+          for (int k = 0; k < HVM_XCHUNK; ++k) xp[k] = c + k < d.nlen ? A.pairs[d.node_off + c + k] : 0ull;
+#pragma unroll
+          for (int k = 0; k < HVM_XCHUNK; ++k) locs[k] = tn[c + k];
+#pragma unroll
+          for (int k = 0; k < HVM_XCHUNK; ++k)
+            if (c + k < d.nlen) net_put<LOC>(A, N, locs[k], adjust_pair(xp[k], tn, tv));
+        }
 #pragma unroll
         for (int c = 0; c < 8; c += 4) {
           u32 vs[4];
@@ -920,8 +944,6 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
           for (int k = 0; k < 4; ++k)
             if ((u32)(c + k) < d.nlen && !((skip_m >> (c + k)) & 1u)) net_put<LOC>(A, N, locs[k], vals[k]);
         }
-        for (u32 i = HVM_PRE_NODE; i < d.nlen; ++i)
-          net_put<LOC>(A, N, tn[i], adjust_pair(A.pairs[d.node_off + i], tn, tv));
         if (FAST && fa) {
           // the slots of the nodes not created go back; s0's holds the
           // SWITCH's (n-1, continuation) node when n > 0
