@@ -14,13 +14,15 @@
 #define HVM_DEQ         32     // per-lane redex deque entries (power of two)
 #endif
 // resident waves per SIMD the HBM turn kernel is compiled for (its registers
-// are capped so that this many fit; LDS allows 7 waves per CU)
+// are capped so that this many fit; LDS allows 8 waves per CU)
 #ifndef HVM_TURN_OCC
 #define HVM_TURN_OCC    2
 #endif
 #define HVM_TBL_NODE    256    // per-wave node-slot table (one step's allocations)
 #define HVM_TBL_VARS    256    // per-wave var-slot table
+#ifndef HVM_TBL_SLACK
 #define HVM_TBL_SLACK   8      // entries past the tables, read ahead and never used
+#endif
 #define HVM_TBL_X       128    // tables of the local-net evaluator's HBM slots
                                // (LEAK and SAVE steps)
 
