@@ -33,6 +33,7 @@ constexpr u64 W_ROOT = (W_ROOT_VAL << 3) | HVM_VAR;
 constexpr u32 W_DEQ = 16;        // per-lane redex deque (power of two)
 constexpr u32 W_TBL = HVM_GPU_DEF_MAX;
 constexpr u32 W_LINK_CAP = 1u << 16;
+constexpr int W_XCHUNK = 8;      // template pairs loaded together at a CALL
 
 __host__ __device__ inline u32 wtag(u64 p) { return (u32)(p & 7u); }
 __host__ __device__ inline u64 wval(u64 p) { return p >> 3; }
@@ -406,11 +407,20 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
             if (t == HVM_VAR) return wport(HVM_VAR, L.tbl_v[off_v + v]);
             return (u64)p;
           };
-          for (u32 i = 0; i < d.vlen; ++i) st(&A.vars[L.tbl_v[off_v + i]], W_NONE);
-          for (u32 i = 0; i < d.nlen; ++i) {
-            const u64 p = A.pairs[d.node_off + i];
-            node_put(A, L.tbl_n[off_n + i], adj(hvm_get_fst(p)), adj(hvm_get_snd(p)));
+          // template pairs eight at a time, each chunk's loads issued
+          // together before its stores, and the var stores last: a wait for
+          // a load also waits for every store issued before it, so loading
+          // one pair per node between the stores cost a store round trip
+          // per node
+          for (u32 c = 0; c < d.nlen; c += W_XCHUNK) {
+            u64 xp[W_XCHUNK];
+#pragma unroll
+            for (int k = 0; k < W_XCHUNK; ++k) xp[k] = c + k < d.nlen ? A.pairs[d.node_off + c + k] : 0ull;
+#pragma unroll
+            for (int k = 0; k < W_XCHUNK; ++k)
+              if (c + k < d.nlen) node_put(A, L.tbl_n[off_n + c + k], adj(hvm_get_fst(xp[k])), adj(hvm_get_snd(xp[k])));
           }
+          for (u32 i = 0; i < d.vlen; ++i) st(&A.vars[L.tbl_v[off_v + i]], W_NONE);
           release_creations();
           for (u32 i = 0; i < d.rlen; ++i) {
             const u64 p = A.pairs[d.rbag_off + i];
@@ -544,8 +554,9 @@ bool HvmWide::create(uint64_t node_cap, uint64_t vars_cap, uint32_t bag_cap, uin
 }
 
 void HvmWide::destroy() {
-  hipFree(a.node); hipFree(a.vars); hipFree(a.bag[0]); hipFree(a.bag[1]);
-  hipFree(a.ctl); hipFree(a.wave_state); hipFree(a.stats);
+  // teardown: a failing free has nothing left to undo
+  (void)hipFree(a.node); (void)hipFree(a.vars); (void)hipFree(a.bag[0]); (void)hipFree(a.bag[1]);
+  (void)hipFree(a.ctl); (void)hipFree(a.wave_state); (void)hipFree(a.stats);
   a = HvmWideArgs{};
 }
 
@@ -598,22 +609,24 @@ int HvmWide::run() {
   return HVM_HIP_OK;
 }
 
+// A failed copy is sticky: later reads return 0 and expand fails.
 unsigned long long HvmWide::vars_load(unsigned long long val) {
   const u64 s = val == W_ROOT_VAL ? 0 : std::min<u64>(val, a.vars_cap);
   u64 v = 0;
-  hipMemcpy(&v, a.vars + s, 8, hipMemcpyDeviceToHost);
+  if (!copy_failed) copy_failed = !ok(hipMemcpy(&v, a.vars + s, 8, hipMemcpyDeviceToHost), "readback copy");
   return v;
 }
 
 void HvmWide::node_load(unsigned long long loc, unsigned long long& f, unsigned long long& g) {
   u64 v[2] = {0, 0};
-  hipMemcpy(v, a.node + 2 * std::min<u64>(loc, a.node_cap), 16, hipMemcpyDeviceToHost);
+  if (!copy_failed)
+    copy_failed = !ok(hipMemcpy(v, a.node + 2 * std::min<u64>(loc, a.node_cap), 16, hipMemcpyDeviceToHost), "readback copy");
   f = v[0];
   g = v[1];
 }
 
 unsigned long long HvmWide::peek(unsigned long long p) {
-  for (int it = 0; it < (1 << 24) && wtag(p) == HVM_VAR; ++it) {
+  for (int it = 0; it < (1 << 24) && wtag(p) == HVM_VAR && !copy_failed; ++it) {
     const u64 v = vars_load(wval(p));
     if (v == W_NONE || v == 0) break;
     p = v;
@@ -631,6 +644,7 @@ int HvmWide::expand(unsigned long long p, unsigned long long& got) {
     if (rc) return rc;
     got = peek(vars_load(W_ROOT_VAL));
   }
+  if (copy_failed) return HVM_HIP_E_DEVICE;
   return ok(hipMemcpy(a.vars, &old, 8, hipMemcpyHostToDevice), "expand") ? HVM_HIP_OK : HVM_HIP_E_DEVICE;
 }
 
@@ -641,6 +655,7 @@ int HvmWide::normalize(const HvmDevDefX* defs, const unsigned long long* pairs, 
   a.ndefs = ndefs;
   deadline = dl;
   turns = launches = 0;
+  copy_failed = false;
   if (!ok(hipMemsetAsync(a.node, 0, ((size_t)a.node_cap + 1) * 16, stream), "clear wide net") ||
       !ok(hipMemsetAsync(a.vars, 0, ((size_t)a.vars_cap + 1) * 8, stream), "clear wide net") ||
       !ok(hipMemsetAsync(a.wave_state, 0, a.nwaves * sizeof(HvmWaveState), stream), "clear wide net") ||
@@ -678,7 +693,8 @@ int HvmWide::normalize(const HvmDevDefX* defs, const unsigned long long* pairs, 
     if (rc == HVM_HIP_OK && !args.empty() && wtag(args[0]) == HVM_CON && tag == 1) {
       u64 m0 = 0, m1 = 0;
       node_load(wval(args[0]), m0, m1);
-      if ((u32)wval(m0) == hvm_new_u24(0xD0CA11) && (u32)wval(m1) == hvm_new_u24(0xFF1FF1)) {
+      if (copy_failed) rc = HVM_HIP_E_DEVICE;
+      else if ((u32)wval(m0) == hvm_new_u24(0xD0CA11) && (u32)wval(m1) == hvm_new_u24(0xFF1FF1)) {
         err = "IO calls run on the 32-bit evaluator (no HVM_HIP_WIDE)";
         rc = HVM_HIP_E_IO;
       }
@@ -715,7 +731,7 @@ bool HvmWide::print_root(const std::vector<std::string>& names, std::string& out
   while (good && (!want_n.empty() || !want_v.empty())) {
     const size_t nn = want_n.size(), nv = want_v.size(), need = std::max<size_t>(nn + nv, 1);
     if (need > cap) {
-      hipFree(d_idx); hipFree(d_out);
+      (void)hipFree(d_idx); (void)hipFree(d_out);
       cap = need * 2;
       good = hipMalloc(&d_idx, cap * 4) == hipSuccess && hipMalloc(&d_out, cap * 16) == hipSuccess;
       if (!good) break;
@@ -744,8 +760,8 @@ bool HvmWide::print_root(const std::vector<std::string>& names, std::string& out
       if (res[2 * nn + i] != W_NONE && res[2 * nn + i] != 0) want_port(res[2 * nn + i]);
     }
   }
-  hipFree(d_idx);
-  hipFree(d_out);
+  (void)hipFree(d_idx);   // scratch of this readback: nothing to undo if a free fails
+  (void)hipFree(d_out);
   if (!good) { err = "wide readback failed"; return false; }
   // print (src/hvm.c:1618-1699)
   out.clear();

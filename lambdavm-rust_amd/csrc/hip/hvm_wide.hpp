@@ -68,6 +68,7 @@ struct HvmWide {
  private:
   double deadline = 0;
   uint32_t turns = 0, launches = 0;
+  bool copy_failed = false;   // a readback copy failed (sticky until the next normalize)
   bool ok(hipError_t e, const char* what);
   bool boot(unsigned long long a_port, unsigned long long b_port);
   int run();
