@@ -52,20 +52,22 @@ int run_probe(uint64_t slots, double* ops_per_s) {
   if (hipMalloc(&sink, sizeof(T)) == hipSuccess && hipMemset(buf, 0, slots * sizeof(T)) == hipSuccess &&
       hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
     const uint32_t mask = (uint32_t)(slots - 1);
+    // a warm-up launch, then the timed one
     hipLaunchKernelGGL(atomic_probe_kernel<T>, dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
-    hipEventRecord(e0, 0);
-    hipLaunchKernelGGL(atomic_probe_kernel<T>, dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
-    hipEventRecord(e1, 0);
+    bool good = hipGetLastError() == hipSuccess && hipEventRecord(e0, 0) == hipSuccess;
+    if (good) hipLaunchKernelGGL(atomic_probe_kernel<T>, dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
+    good = good && hipGetLastError() == hipSuccess && hipEventRecord(e1, 0) == hipSuccess;
     float ms = 0;
-    if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0) {
+    if (good && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0) {
       *ops_per_s = (double)threads * blocks * rounds * HVM_PROBE_INFLIGHT / (ms * 1e-3);
       rc = HVM_HIP_OK;
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  if (sink) hipFree(sink);
-  hipFree(buf);
+  // teardown: a failing destroy or free has nothing left to undo
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (sink) (void)hipFree(sink);
+  (void)hipFree(buf);
   return rc;
 }
 

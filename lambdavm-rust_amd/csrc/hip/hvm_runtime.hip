@@ -169,24 +169,16 @@ struct hvm_hip_ctx {
 
   void release() {
     if (wide) { wide->destroy(); delete wide; wide = nullptr; }
-    if (node) hipFree(node);
-    if (vars) hipFree(vars);
-    if (bag[0]) hipFree(bag[0]);
-    if (bag[1]) hipFree(bag[1]);
-    if (ctl) hipFree(ctl);
-    if (wave_state) hipFree(wave_state);
-    if (lspill) hipFree(lspill);
-    if (stats) hipFree(stats);
-    if (sched) hipFree(sched);
-    if (d_params) hipFree(d_params);
-    if (d_trace) hipFree(d_trace);
-    if (d_defs) hipFree(d_defs);
-    if (d_pairs) hipFree(d_pairs);
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
+    // teardown: a failing free or destroy has nothing left to undo
+    for (void* p : {(void*)node, (void*)vars, (void*)bag[0], (void*)bag[1], (void*)ctl, (void*)wave_state,
+                    (void*)lspill, (void*)stats, (void*)sched, (void*)d_params, (void*)d_trace, (void*)d_defs,
+                    (void*)d_pairs})
+      if (p) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
     drop_graphs();
-    for (hipEvent_t e : tev) hipEventDestroy(e);
-    if (stream) hipStreamDestroy(stream);
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 
   bool load(const u32* buf, size_t n_words) {
@@ -255,8 +247,8 @@ struct hvm_hip_ctx {
       for (u32 i = 0; i < HVM_PRE_NODE && i < x.h.nlen; ++i) x.node[i] = pairs[x.h.node_off + i];
       for (u32 i = 0; i < HVM_PRE_RBAG && i < x.h.rlen; ++i) x.rbag[i] = pairs[x.h.rbag_off + i];
     }
-    if (d_defs) { hipFree(d_defs); d_defs = nullptr; }
-    if (d_pairs) { hipFree(d_pairs); d_pairs = nullptr; }
+    if (d_defs) { (void)hipFree(d_defs); d_defs = nullptr; }
+    if (d_pairs) { (void)hipFree(d_pairs); d_pairs = nullptr; }
     if (!ck(hipMalloc(&d_defs, defx.size() * sizeof(HvmDevDefX)), "hipMalloc(defs)")) return false;
     if (!ck(hipMalloc(&d_pairs, pairs.size() * 8), "hipMalloc(pairs)")) return false;
     if (!ck(hipMemcpy(d_defs, defx.data(), defx.size() * sizeof(HvmDevDefX), hipMemcpyHostToDevice), "copy defs")) return false;
@@ -268,24 +260,30 @@ struct hvm_hip_ctx {
   }
 
   // --- small host accessors (the reference's gnet_* helpers, src/hvm.cu:1842-1881)
+  // A failed copy is sticky: later reads return 0 and the readback that
+  // used them fails (expand and the IO driver check copy_failed).
+  bool copy_failed = false;
+  void copy_ck(hipError_t e) {
+    if (e != hipSuccess && !copy_failed) { copy_failed = true; ck(e, "readback copy"); }
+  }
   u64 node_load(u32 loc) {
     u64 v = 0;
-    if (loc < node_cap) hipMemcpy(&v, node + loc, 8, hipMemcpyDeviceToHost);
+    if (loc < node_cap && !copy_failed) copy_ck(hipMemcpy(&v, node + loc, 8, hipMemcpyDeviceToHost));
     return v;
   }
   u32 vidx(u32 val) const { return val == HVM_ROOT_VAL ? 0u : val; }
   u32 vars_load(u32 val) {
     u32 v = 0;
     u32 i = vidx(val);
-    if (i < vars_cap) hipMemcpy(&v, vars + i, 4, hipMemcpyDeviceToHost);
+    if (i < vars_cap && !copy_failed) copy_ck(hipMemcpy(&v, vars + i, 4, hipMemcpyDeviceToHost));
     return v;
   }
   void vars_store(u32 val, u32 x) {
     u32 i = vidx(val);
-    if (i < vars_cap) hipMemcpy(vars + i, &x, 4, hipMemcpyHostToDevice);
+    if (i < vars_cap && !copy_failed) copy_ck(hipMemcpy(vars + i, &x, 4, hipMemcpyHostToDevice));
   }
   u32 peek(u32 port) {
-    for (int it = 0; it < (1 << 24) && hvm_get_tag(port) == HVM_VAR; ++it) {
+    for (int it = 0; it < (1 << 24) && hvm_get_tag(port) == HVM_VAR && !copy_failed; ++it) {
       u32 v = vars_load(hvm_get_val(port));
       if (v == HVM_NONE || v == 0) break;
       port = v;
@@ -302,6 +300,7 @@ struct hvm_hip_ctx {
   }
 
   bool reset_net() {
+    copy_failed = false;
     if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return false;
     return reset_heap();
   }
@@ -327,7 +326,7 @@ struct hvm_hip_ctx {
     if (params && !ck(hipMemcpy(d_params, params, HVM_MLP_PARAMS * sizeof(unsigned short), hipMemcpyHostToDevice),
                       "params upload")) return false;
     if (trace_cap != host_sched.trace_cap) {
-      if (d_trace) { hipFree(d_trace); d_trace = nullptr; }
+      if (d_trace) { (void)hipFree(d_trace); d_trace = nullptr; }
       if (trace_cap && !ck(hipMalloc(&d_trace, (size_t)trace_cap * HVM_TRACE_REC * sizeof(float)), "hipMalloc(trace)"))
         return false;
     }
@@ -384,7 +383,7 @@ struct hvm_hip_ctx {
 
   void drop_graphs() {
     for (hipGraphExec_t& g : chunk_graph) {
-      if (g) hipGraphExecDestroy(g);
+      if (g) (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
   }
@@ -394,9 +393,9 @@ struct hvm_hip_ctx {
     const bool graphs = (cfg.flags & HVM_HIP_GRAPHS) != 0;
     auto direct = [&](hipStream_t st) {
       for (u32 i = 0; i < chunk; ++i) {
-        if (!graphs) hipEventRecord(tev[2 * i], st);
+        if (!graphs && !ck(hipEventRecord(tev[2 * i], st), "record turn start")) return false;
         if (!ck(hvm_launch_turn(&a, st), "launch turn")) return false;
-        if (!graphs) hipEventRecord(tev[2 * i + 1], st);
+        if (!graphs && !ck(hipEventRecord(tev[2 * i + 1], st), "record turn end")) return false;
         if (!ck(hvm_launch_between(&a, clock_khz, st), "launch between")) return false;
       }
       return true;
@@ -413,14 +412,14 @@ struct hvm_hip_ctx {
       if (!ck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
       bool ok = direct(stream);
       hipError_t e = hipStreamEndCapture(stream, &g);
-      if (!ok) { if (g) hipGraphDestroy(g); return false; }
+      if (!ok) { if (g) (void)hipGraphDestroy(g); return false; }
       if (!ck(e, "end capture")) return false;
       e = hipGraphInstantiate(&chunk_graph[k], g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
+      (void)hipGraphDestroy(g);   // the instantiated graph no longer needs it
       if (!ck(e, "instantiate graph")) return false;
     }
     // the graph holds kernels only; the chunk is timed as a whole
-    hipEventRecord(tev[0], stream);
+    if (!ck(hipEventRecord(tev[0], stream), "record chunk start")) return false;
     if (!ck(hipGraphLaunch(chunk_graph[k], stream), "launch graph")) return false;
     return ck(hipEventRecord(tev[1], stream), "record chunk end");
   }
@@ -481,13 +480,13 @@ struct hvm_hip_ctx {
       if (cfg.flags & HVM_HIP_GRAPHS) {
         // one span for the chunk (its between kernels included)
         float ms = 0;
-        hipEventElapsedTime(&ms, tev[0], tev[1]);
+        if (!ck(hipEventElapsedTime(&ms, tev[0], tev[1]), "chunk time")) return false;
         kernel_ms += ms * (float)ran / (float)chunk;
         launch_ms += ms;
       } else {
         for (u32 i = 0; i < chunk; ++i) {
           float ms = 0;
-          hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]);
+          if (!ck(hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]), "turn time")) return false;
           if (i < ran) kernel_ms += ms;
           launch_ms += ms;
         }
@@ -517,7 +516,7 @@ struct hvm_hip_ctx {
       got = peek(vars_load(HVM_ROOT_VAL));
     }
     vars_store(HVM_ROOT_VAL, old);
-    return true;
+    return !copy_failed;
   }
 
   struct Ctr { int32_t tag = -1; u32 args_len = 0; u32 args[16]; };
@@ -589,7 +588,7 @@ struct hvm_hip_ctx {
       if (hnode[hnode_cur] == 0) {
         u32 loc = node_pool_hi + hnode_cur;
         hnode[hnode_cur] = val;
-        hipMemcpy(node + loc, &val, 8, hipMemcpyHostToDevice);
+        copy_ck(hipMemcpy(node + loc, &val, 8, hipMemcpyHostToDevice));
         return loc;
       }
     }
@@ -787,6 +786,7 @@ struct hvm_hip_ctx {
       if (!readback_ctr(peek(port), c)) return HVM_HIP_E_EVAL;
       if (c.args_len < 1 || hvm_get_tag(c.args[0]) != HVM_CON) break;
       u64 magic = node_load(hvm_get_val(c.args[0]));
+      if (copy_failed) return HVM_HIP_E_DEVICE;
       if (hvm_get_val(hvm_get_fst(magic)) != hvm_new_u24(0xD0CA11) ||
           hvm_get_val(hvm_get_snd(magic)) != hvm_new_u24(0xFF1FF1)) break;
       if (c.tag != 1) break;                       // IO/Done (or not an IO call)
@@ -802,6 +802,7 @@ struct hvm_hip_ctx {
       u32 ret = 0;
       if (!io_call(name, argm, ret)) return HVM_HIP_E_EVAL;
       u32 app = io_full ? 0 : host_node(hvm_new_pair(ret, HVM_ROOT));
+      if (copy_failed) return HVM_HIP_E_DEVICE;
       if (io_full || !app) { fail("IO call results exceed the host region of the net"); return HVM_HIP_E_IO; }
       if (!boot(hvm_new_pair(hvm_new_port(HVM_CON, app), cont))) return HVM_HIP_E_DEVICE;
       port = HVM_ROOT;
@@ -825,8 +826,8 @@ struct hvm_hip_ctx {
       u32 cnt[3] = {0, 0, 0};
       std::vector<u32> nidx, vidx_, vval;
       std::vector<u64> nval;
+      if (ok) ok = ck(hipMemsetAsync(d_cnt, 0, 16, stream), "gather counts clear");
       if (ok) {
-        hipMemsetAsync(d_cnt, 0, 16, stream);
         HvmGatherArgs g;
         g.node = node; g.vars = vars; g.node_cap = node_cap; g.vars_cap = vars_cap;
         g.root_port = port;
@@ -844,7 +845,9 @@ struct hvm_hip_ctx {
                ck(hipMemcpy(vval.data(), d_vval, (size_t)cnt[1] * 4, hipMemcpyDeviceToHost), "gather copy");
         }
       }
-      hipFree(d_nidx); hipFree(d_nval); hipFree(d_vidx); hipFree(d_vval); hipFree(d_q); hipFree(d_cnt);
+      // scratch of this readback: nothing to undo if a free fails
+      (void)hipFree(d_nidx); (void)hipFree(d_nval); (void)hipFree(d_vidx); (void)hipFree(d_vval);
+      (void)hipFree(d_q); (void)hipFree(d_cnt);
       if (!ok) return false;
       if (cnt[2] & 1) return fail("readback reached a port outside the net");
       if (cnt[2] & 2 || cnt[0] > cap || cnt[1] > cap) continue;  // grow and retry
@@ -961,7 +964,7 @@ struct hvm_hip_ctx {
     double t_eval = now_s();
     HvmDevStats s;
     memset(&s, 0, sizeof s);
-    hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost);
+    if (!ck(hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost), "stats copy") && rc == 0) rc = HVM_HIP_E_DEVICE;
     if (out) {
       memset(out, 0, sizeof *out);
       out->itrs = s.itrs;
@@ -981,7 +984,7 @@ struct hvm_hip_ctx {
     }
     if (rc != 0) {
       u32 e = 0;
-      hipMemcpy(&e, &ctl->err, 4, hipMemcpyDeviceToHost);
+      if (hipMemcpy(&e, &ctl->err, 4, hipMemcpyDeviceToHost) != hipSuccess) e = 0;   // the flags only add detail
       if (out) out->err = e;
       return rc;
     }
@@ -1024,10 +1027,10 @@ struct hvm_hip_ctx {
     return ck(hipStreamSynchronize(stream), "boot sync");
   }
 
-  void fill_out(hvm_hip_out* out, double t0, double t_eval) {
+  bool fill_out(hvm_hip_out* out, double t0, double t_eval) {
     HvmDevStats s;
     memset(&s, 0, sizeof s);
-    hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost);
+    const bool good = ck(hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost), "stats copy");
     memset(out, 0, sizeof *out);
     out->itrs = s.itrs;
     for (int k = 0; k < HVM_HIP_KINDS; ++k) out->kind[k] = s.kind[k];
@@ -1043,6 +1046,7 @@ struct hvm_hip_ctx {
     out->waves = nwaves;
     out->leak = s.leaks; out->saves = s.saves; out->local_left = s.local_left;
     out->normalizes = normalizes;
+    return good;
   }
 
   int eval_batch(const u32* entries, u32 n, const uint64_t* est, hvm_hip_out* outs, hvm_hip_out* total) {
@@ -1084,6 +1088,7 @@ struct hvm_hip_ctx {
         if (!readback_ctr(peek(root), c)) return HVM_HIP_E_EVAL;
         if (c.args_len >= 1 && hvm_get_tag(c.args[0]) == HVM_CON && c.tag == 1 && c.args_len == 4) {
           u64 magic = node_load(hvm_get_val(c.args[0]));
+          if (copy_failed) return HVM_HIP_E_DEVICE;
           if (hvm_get_val(hvm_get_fst(magic)) == hvm_new_u24(0xD0CA11) &&
               hvm_get_val(hvm_get_snd(magic)) == hvm_new_u24(0xFF1FF1)) {
             outs[j].err = HVM_HIP_E_IO;
@@ -1101,7 +1106,7 @@ struct hvm_hip_ctx {
       start = end;
     }
     if (total) {
-      fill_out(total, t0, now_s());
+      if (!fill_out(total, t0, now_s())) return HVM_HIP_E_DEVICE;
       total->wall_s = now_s() - t0;
     }
     return HVM_HIP_OK;
