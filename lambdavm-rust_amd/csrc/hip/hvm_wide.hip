@@ -421,10 +421,22 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
               if (c + k < d.nlen) node_put(A, L.tbl_n[off_n + c + k], adj(hvm_get_fst(xp[k])), adj(hvm_get_snd(xp[k])));
           }
           for (u32 i = 0; i < d.vlen; ++i) st(&A.vars[L.tbl_v[off_v + i]], W_NONE);
+          // the first redex pairs load while the creation stores complete
+          u64 xr[W_XCHUNK];
+#pragma unroll
+          for (int k = 0; k < W_XCHUNK; ++k) xr[k] = (u32)k < d.rlen ? A.pairs[d.rbag_off + k] : 0ull;
           release_creations();
-          for (u32 i = 0; i < d.rlen; ++i) {
-            const u64 p = A.pairs[d.rbag_off + i];
-            link(L, ln, A, adj(hvm_get_fst(p)), adj(hvm_get_snd(p)));
+          for (u32 c = 0; c < d.rlen; c += W_XCHUNK) {
+            if (c) {
+#pragma unroll
+              for (int k = 0; k < W_XCHUNK; ++k) xr[k] = c + k < d.rlen ? A.pairs[d.rbag_off + c + k] : 0ull;
+            }
+            const u32 m = min(d.rlen - c, (u32)W_XCHUNK);
+            for (u32 k = 0; k < m; ++k) {
+              link(L, ln, A, adj(hvm_get_fst(xr[0])), adj(hvm_get_snd(xr[0])));
+#pragma unroll
+              for (int j = 0; j + 1 < W_XCHUNK; ++j) xr[j] = xr[j + 1];
+            }
           }
           link(L, ln, A, adj(d.root), b);
           kk = HVM_K_CALL; kb = d.call_bytes;
