@@ -29,15 +29,16 @@ from typing import List, Sequence, Tuple
 import hvm_hip as H
 
 
-def collect_traces(words, seeds: Sequence[int], explore_pct: int = 35, **cfg):
-    """Runs the program once per seed in explore mode; returns the decision
-    records that have a measured reward."""
+def collect_traces(words, seeds: Sequence[int], explore_pct: int = 35, run=None, **cfg):
+    """Runs the program once per seed in explore mode (`run(ctx)` evaluates
+    it; default ctx.eval()); returns the decision records that have a
+    measured reward."""
     recs = []
     with H.Context(flags=H.NO_READBACK, **cfg) as ctx:
         ctx.load(words)
         for s in seeds:
             ctx.set_scheduler(H.SCHED_EXPLORE, seed=s, explore_pct=explore_pct, trace_cap=1 << 16)
-            ctx.eval()
+            (run or (lambda c: c.eval()))(ctx)
             for feats, action, reward, itrs, us in ctx.read_trace():
                 if us > 0:
                     recs.append((feats, action, reward))
