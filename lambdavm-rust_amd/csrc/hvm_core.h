@@ -16,10 +16,14 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define HVM_HD __host__ __device__ __forceinline__
+// rarely taken code kept out of line: its constants stay out of the caller's
+// registers (the f24 operators' libm constants would otherwise be spilled)
+#define HVM_HD_COLD static __host__ __device__ __attribute__((noinline))
 #else
 #include <math.h>
 #include <string.h>
 #define HVM_HD static inline
+#define HVM_HD_COLD static
 #endif
 
 typedef uint8_t  hvm_tag;
@@ -195,6 +199,31 @@ HVM_HD hvm_numb hvm_cast(hvm_numb a, hvm_numb b) {
   return hvm_new_u24(0);
 }
 
+// The f24 operators of hvm_operate (src/hvm.c:559-579).
+HVM_HD_COLD hvm_numb hvm_operate_f24(hvm_tag op, hvm_numb a, hvm_numb b) {
+  float av = hvm_get_f24(a), bv = hvm_get_f24(b);
+  switch (op) {
+    case HVM_OP_ADD: return hvm_new_f24(av + bv);
+    case HVM_OP_SUB: return hvm_new_f24(av - bv);
+    case HVM_FP_SUB: return hvm_new_f24(bv - av);
+    case HVM_OP_MUL: return hvm_new_f24(av * bv);
+    case HVM_OP_DIV: return hvm_new_f24(av / bv);
+    case HVM_FP_DIV: return hvm_new_f24(bv / av);
+    case HVM_OP_REM: return hvm_new_f24(fmodf(av, bv));
+    case HVM_FP_REM: return hvm_new_f24(fmodf(bv, av));
+    case HVM_OP_EQ:  return hvm_new_u24(av == bv);
+    case HVM_OP_NEQ: return hvm_new_u24(av != bv);
+    case HVM_OP_LT:  return hvm_new_u24(av < bv);
+    case HVM_OP_GT:  return hvm_new_u24(av > bv);
+    case HVM_OP_AND: return hvm_new_f24(atan2f(av, bv));
+    case HVM_OP_OR:  return hvm_new_f24(logf(bv) / logf(av));
+    case HVM_OP_XOR: return hvm_new_f24(powf(av, bv));
+    case HVM_OP_SHL: return hvm_new_f24((float)sin((double)(av + bv)));
+    case HVM_OP_SHR: return hvm_new_f24((float)tan((double)(av + bv)));
+    default:         return hvm_new_f24(0.0f);
+  }
+}
+
 // The numeric ALU (src/hvm.c:475-582).  u24/i24 division or remainder by zero
 // is undefined in the reference C runtime; here it yields 0 instead of trapping.
 HVM_HD hvm_numb hvm_operate(hvm_numb a, hvm_numb b) {
@@ -260,28 +289,6 @@ HVM_HD hvm_numb hvm_operate(hvm_numb a, hvm_numb b) {
       default:         return hvm_new_i24(0);
     }
   }
-  if (ty == HVM_TY_F24) {
-    float av = hvm_get_f24(a), bv = hvm_get_f24(b);
-    switch (op) {
-      case HVM_OP_ADD: return hvm_new_f24(av + bv);
-      case HVM_OP_SUB: return hvm_new_f24(av - bv);
-      case HVM_FP_SUB: return hvm_new_f24(bv - av);
-      case HVM_OP_MUL: return hvm_new_f24(av * bv);
-      case HVM_OP_DIV: return hvm_new_f24(av / bv);
-      case HVM_FP_DIV: return hvm_new_f24(bv / av);
-      case HVM_OP_REM: return hvm_new_f24(fmodf(av, bv));
-      case HVM_FP_REM: return hvm_new_f24(fmodf(bv, av));
-      case HVM_OP_EQ:  return hvm_new_u24(av == bv);
-      case HVM_OP_NEQ: return hvm_new_u24(av != bv);
-      case HVM_OP_LT:  return hvm_new_u24(av < bv);
-      case HVM_OP_GT:  return hvm_new_u24(av > bv);
-      case HVM_OP_AND: return hvm_new_f24(atan2f(av, bv));
-      case HVM_OP_OR:  return hvm_new_f24(logf(bv) / logf(av));
-      case HVM_OP_XOR: return hvm_new_f24(powf(av, bv));
-      case HVM_OP_SHL: return hvm_new_f24((float)sin((double)(av + bv)));
-      case HVM_OP_SHR: return hvm_new_f24((float)tan((double)(av + bv)));
-      default:         return hvm_new_f24(0.0f);
-    }
-  }
+  if (ty == HVM_TY_F24) return hvm_operate_f24(op, a, b);
   return hvm_new_u24(0);
 }
