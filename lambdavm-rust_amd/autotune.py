@@ -23,7 +23,10 @@ Search: coordinate descent from the defaults, one knob at a time, keeping a
 value only when its median time beats the best so far.  Every candidate must
 reproduce the workload's interaction count (ITRS) and, for single nets, the
 canonical normal form; a candidate that does not is an error, not a slow
-point.  The result is written as JSON: the chosen configuration, the default
+point, and so is an evaluator error flag (it is raised, not skipped); only
+candidates that the library rejects at context creation or book load, or
+that run out of the time limit, are recorded as not runnable.  The result is
+written as JSON: the chosen configuration, the default
 and tuned times, and every measurement.
 """
 from __future__ import annotations
@@ -108,8 +111,17 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
                 continue
             try:
                 t, ident2, _ = measure(w, cand, repeats)
+            except H.HvmEvalError as e:
+                # an evaluator error flag is a failure of the evaluator, never
+                # a slow point; only running out of the time limit is skipped
+                if e.err or "time limit" not in str(e):
+                    raise
+                log(f"  {knob}={v}: over the time limit ({e})")
+                rows.append({"cfg": dict(cand), "error": str(e)})
+                continue
             except H.HvmError as e:
-                # e.g. a net too large for the local-net evaluator's 2^28 cap
+                # rejected at context creation or book load: e.g. a net too
+                # large for the local-net evaluator's 2^28 cap
                 log(f"  {knob}={v}: not runnable ({e})")
                 rows.append({"cfg": dict(cand), "error": str(e)})
                 continue

@@ -358,10 +358,14 @@ __device__ __forceinline__ u32 scan_idx(const Scan& s, u32 lane, u32& rel) {
 // Processes one 64-slot window (value v at idx for this lane).  Records free
 // slots into tbl[got...]; when the window satisfies the request the lane
 // holding the last slot taken writes the next cursor to *cursor_slot.
+// A scan covers each slot of its range at most once: the slots it has handed
+// out in this call are not created until the creation phase, so they still
+// read zero, and a window that wrapped around onto them would hand them out a
+// second time (two interactions writing one slot).
 template <typename T>
 __device__ __forceinline__ void scan_window(T* buf, Scan& s, T v, u32 idx, u32 rel, u32 reserved, T claim,
                                             u32* tbl, u32* cursor_slot) {
-  bool cand = (v == 0) && (idx >= reserved);
+  bool cand = (v == 0) && (idx >= reserved) && (s.scanned + lane_id() < s.len);
   if (s.pool && cand) cand = atomicCAS(&buf[idx], (T)0, claim) == (T)0;
   u64 m = __ballot(cand);
   u32 r = rank_in(m);
@@ -381,9 +385,10 @@ __device__ __forceinline__ void scan_window(T* buf, Scan& s, T v, u32 idx, u32 r
 }
 
 // After a window that left the request open: moves from an exhausted slice
-// to the pool; returns false when the pool is exhausted too.
+// (every slot looked at once) to the pool; returns false when the pool is
+// exhausted too.
 __device__ __forceinline__ bool scan_advance(Scan& s, u32 pool_lo, u32 pool_len, u32 pool_cur) {
-  if (s.got >= s.need || s.scanned < s.len + HVM_WAVE) return true;
+  if (s.got >= s.need || s.scanned < s.len) return true;
   if (s.pool || pool_len < HVM_WAVE) return false;
   s.pool = true;
   s.lo = pool_lo;
@@ -672,11 +677,19 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     // the interaction commits, so a lane that retries has removed nothing.
     // The local-net evaluator takes after the reservation (P3).
     constexpr bool EARLY = !LOC && HVM_EARLY_TAKES;
-    constexpr bool PEEK = EARLY && HVM_PEEK_TAKES;
-    auto hbm_take = [](const HvmTurnArgs& A, u32 loc) { return PEEK ? node_peek(A, loc) : node_take(A, loc); };
     const bool take_a0 = has && (rule == HVM_ANNI || rule == HVM_COMM);
     const bool take_b0 = has && (rule == HVM_ERAS || rule == HVM_ANNI || rule == HVM_COMM ||
                                  rule == HVM_OPER || rule == HVM_SWIT || (LOC && rule == HVM_LEAK));
+    // Lanes with two takes (ANNI, COMM) always read their nodes with loads
+    // and zero the slots once the interaction commits.  An early take by
+    // exchange frees the slot at once, so the owning wave's allocator may
+    // hand it out in this very step; if the lane's other node were then not
+    // there, putting the first one back would overwrite the new node.  With
+    // loads nothing is removed before the commit, and a lane that retries
+    // has nothing to put back (src/hvm.c:921-924 checks both nodes before
+    // taking either for the same reason).
+    const bool PEEK = EARLY && (HVM_PEEK_TAKES || take_a0);
+    auto hbm_take = [PEEK](const HvmTurnArgs& A, u32 loc) { return PEEK ? node_peek(A, loc) : node_take(A, loc); };
     // slots from the net's main allocator (local with LOC), known here for
     // every rule but CALL
     u32 nn = 0, nv = 0;
@@ -839,7 +852,9 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
 
     if ((take_a && NA == 0) || (take_b && NB == 0)) {
       // A node this redex points to is not visible yet: put everything back
-      // and retry the redex on a later step (src/hvm.c:1112-1114).
+      // and retry the redex on a later step (src/hvm.c:1112-1114).  In the
+      // HBM evaluator nothing taken needs to go back: a lane with two takes
+      // read its nodes with loads, and a lane with one take found nothing.
       if ((take_a && NA == 0 && !is_loc_val(hvm_get_val(a)) && hvm_get_val(a) >= A.node_cap) ||
           (take_b && NB == 0 && !is_loc_val(hvm_get_val(b)) && hvm_get_val(b) >= A.node_cap))
         atomicOr(&A.ctl->err, (u32)HVM_ERR_BAD_PORT);   // not a node of this net

@@ -49,6 +49,15 @@ class HvmError(RuntimeError):
     pass
 
 
+class HvmEvalError(HvmError):
+    """The evaluator stopped: error flags (`err`, HVM_ERR_* bits) or its time
+    limit.  Context creation and book loading raise plain HvmError."""
+
+    def __init__(self, msg: str, err: int = 0):
+        super().__init__(msg)
+        self.err = err
+
+
 class _Cfg(C.Structure):
     _fields_ = [("node_cap", C.c_uint64), ("vars_cap", C.c_uint64), ("bag_cap", C.c_uint32),
                 ("waves", C.c_uint32), ("turn_steps", C.c_uint32), ("flags", C.c_uint32),
@@ -239,7 +248,8 @@ class Context:
         rc = lib().hvm_hip_eval(self._h, C.byref(o))
         r = _result(rc, o)
         if rc != 0:
-            raise HvmError(f"eval failed ({STATUS.get(rc, rc)}, err=0x{r.err:x}): " + self.last_error())
+            msg = f"eval failed ({STATUS.get(rc, rc)}, err=0x{r.err:x}): " + self.last_error()
+            raise (HvmEvalError(msg, r.err) if rc == 4 else HvmError(msg))
         return r
 
     def eval_batch(self, entries, est_nodes=None):
@@ -255,7 +265,8 @@ class Context:
         per = [_result(outs[i].err, outs[i]) for i in range(n)]
         tot = _result(rc, total)
         if rc != 0:
-            raise HvmError(f"eval_batch failed ({STATUS.get(rc, rc)}, err=0x{tot.err:x}): " + self.last_error())
+            msg = f"eval_batch failed ({STATUS.get(rc, rc)}, err=0x{tot.err:x}): " + self.last_error()
+            raise (HvmEvalError(msg, tot.err) if rc == 4 else HvmError(msg))
         return per, tot
 
     def set_scheduler(self, mode: int = SCHED_PLAIN, seed: int = 0, explore_pct: int = 0,
@@ -315,7 +326,8 @@ def run_hip(words, **cfg) -> RunResult:
     rc = L.hvm_hip_run(arr, len(words), C.byref(c), C.byref(o))
     r = _result(rc, o)
     if rc != 0:
-        raise HvmError(f"hvm_hip_run failed: {STATUS.get(rc, rc)} (err=0x{r.err:x})")
+        msg = f"hvm_hip_run failed: {STATUS.get(rc, rc)} (err=0x{r.err:x})"
+        raise (HvmEvalError(msg, r.err) if rc == 4 else HvmError(msg))
     return r
 
 

@@ -21,6 +21,17 @@ Programs (all names and encodings here are this project's own):
 * `church(n, m)`    — 2^n independent iterations, each applying the Church
                       numeral 2^m (built by multiplying numerals) to a u24
                       increment starting at 0, summed: result 2^(n+m) mod 2^24.
+* `radix_lsd(k, seed)` — LSD radix sort of 2^k uniform u24 keys (duplicates
+                      kept), one bit per pass over 24 passes: the keys become a
+                      rope (Empty/Single/Concat, Scott-encoded) over the depth-k
+                      key tree; each pass is a tree map of a key classifier
+                      (a closure copied at every node) with a stable merge of
+                      the zeros and ones ropes; then a parallel checksum
+                      Σ sorted[i]·i mod 2^24.
+* `church_loop(n, m)` — an outer loop of 2^n iterations; iteration i applies
+                      the Church numeral 2^m to a u24 increment starting at i
+                      and adds the result to the loop's accumulator:
+                      Σ_i (i + 2^m) mod 2^24.
 """
 from __future__ import annotations
 
@@ -264,6 +275,122 @@ def radix(k: int = 18, seed: int = 0) -> Tuple[str, int]:
     return code, radix_checksum(keys)
 
 
+# LSD radix sort over ropes.  A rope is Scott-encoded: Empty = λe λs λc. e,
+# Single k = λe λs λc. s k, Concat l x = λe λs λc. c l x.  `@cat` joins two
+# ropes and drops Empty sides (so a rope keeps exactly one Single per key);
+# `@split f t` maps the classifier f (key -> 0/1) over t and merges the
+# results into the pair (zeros, ones), keeping the order within each side —
+# the stable partition of one LSD pass; `@bitf b` is the classifier of bit b.
+# `@lsd n b t` runs n passes from bit b; `@rope d t` turns a depth-d key tree
+# into a rope; `@cks t` returns (count (sum w)) with w = Σ key_j * j, combined
+# over the two halves of a Concat as w1 + w2 + sum2 * count1.
+LSD_DEFS = """
+@E = (e (* (* e)))
+@S = (k (* ((k r) (* r))))
+@C = (l (x (* (* ((l (x r)) r)))))
+@cat = (x (y r)) & x ~ (@ca_e (@ca_s (@ca_c (y r))))
+@ca_e = (y y)
+@ca_s = (k (y r)) & y ~ (@cb_e (@cb_s (@cb_c (k r))))
+@cb_e = (k r) & @S ~ (k r)
+@cb_s = (j (k r)) & @S ~ (k a) & @S ~ (j b) & @C ~ (a (b r))
+@cb_c = (l (x (k r))) & @S ~ (k a) & @C ~ (l (x b)) & @C ~ (a (b r))
+@ca_c = (l (x (y r))) & y ~ (@cc_e (@cc_s (@cc_c (l (x r)))))
+@cc_e = (l (x r)) & @C ~ (l (x r))
+@cc_s = (j (l (x r))) & @C ~ (l (x a)) & @S ~ (j b) & @C ~ (a (b r))
+@cc_c = (l2 (x2 (l (x r)))) & @C ~ (l (x a)) & @C ~ (l2 (x2 b)) & @C ~ (a (b r))
+@split = (f (t r)) & t ~ (@sp_e (@sp_s (@sp_c (f r))))
+@sp_e = (* (@E @E))
+@sp_s = ({k1 k2} (f r)) & f ~ (k1 bit) & bit ~ ?((@sp_0 @sp_1) (k2 r))
+@sp_0 = (k (s @E)) & @S ~ (k s)
+@sp_1 = (* (k (@E s))) & @S ~ (k s)
+@sp_c = (l (x ({f1 f2} (z o))))
+  &! @split ~ (f1 (l (zl ol)))
+  &! @split ~ (f2 (x (zx ox)))
+  & @cat ~ (zl (zx z))
+  & @cat ~ (ol (ox o))
+@bitf = (b (k r)) & k ~ $([>>] $(b sh)) & sh ~ $([&] $(1 r))
+@lsd = (?((@lsd_z @lsd_s) (b (t r))) (b (t r)))
+@lsd_z = (* (t t))
+@lsd_s = (n ({b1 b2} (t r)))
+  & @bitf ~ (b1 f)
+  & @split ~ (f (t (z o)))
+  & @cat ~ (z (o t2))
+  & b2 ~ $([+1] b3)
+  & @lsd ~ (n (b3 (t2 r)))
+@rope = (?((@ro_z @ro_s) (t r)) (t r))
+@ro_z = (k r) & @S ~ (k r)
+@ro_s = ({d1 d2} ((l u) r))
+  &! @rope ~ (d1 (l a))
+  &! @rope ~ (d2 (u b))
+  & @C ~ (a (b r))
+@cks = (t r) & t ~ (@ck_e (@ck_s (@ck_c r)))
+@ck_e = (0 (0 0))
+@ck_s = (k (1 (k 0)))
+@ck_c = (l (x (c (s w))))
+  &! @cks ~ (l ({c1a c1b} (s1 w1)))
+  &! @cks ~ (x (c2 ({s2a s2b} w2)))
+  & c1a ~ $([+] $(c2 c))
+  & s1 ~ $([+] $(s2a s))
+  & s2b ~ $([*] $(c1b t))
+  & w1 ~ $([+] $(w2 u))
+  & u ~ $([+] $(t w))
+"""
+
+
+def lsd_checksum(keys: List[int]) -> int:
+    return sum(k * i for i, k in enumerate(sorted(keys))) % U24
+
+
+def radix_lsd(k: int = 18, seed: int = 0, bits: int = 24) -> Tuple[str, int]:
+    """LSD radix sort of 2^k uniform u24 keys (with duplicates), `bits`
+    one-bit passes; result Σ sorted[i]·i mod 2^24."""
+    keys = keys_uniform(1 << k, seed)
+    top, lines = key_tree_defs(keys, "keys")
+    code = (f"@main = w & @rope ~ ({k} (@{top} t)) & @lsd ~ ({bits} (0 (t t2))) & @cks ~ (t2 (* (* w)))\n"
+            + LSD_DEFS + "\n".join(lines) + "\n")
+    if bits == 24:
+        return code, lsd_checksum(keys)
+    part = sorted(keys, key=lambda x: x & ((1 << bits) - 1))   # stable: by the low bits
+    return code, sum(x * i for i, x in enumerate(part)) % U24
+
+
+def church_numeral_defs(m: int) -> Tuple[List[str], str]:
+    """Defs of the Church numeral 2^m, built by multiplying numerals (2^1,
+    2^2, 2^4, ... by squaring, then the binary decomposition of m); returns
+    the lines and the numeral's def name."""
+    assert m >= 1
+    lines = ["@inc = ($([+1] r) r)",
+             "@c1 = ({(x y) (y r)} (x r))",
+             "@mul = (f (g (h r))) & g ~ (h i) & f ~ (i r)"]
+    have = {1: "c1"}
+    e = 1
+    while e * 2 <= m:
+        lines.append(f"@c{e * 2} = r & @mul ~ (@c{e} (@c{e} r))")
+        have[e * 2] = f"c{e * 2}"
+        e *= 2
+    parts = [p for p in sorted(have) if m & p]
+    if sum(parts) != m:
+        raise ValueError("m must be a sum of distinct powers of two")
+    cur = have[parts[0]]
+    for p in parts[1:]:
+        nxt = f"{cur}x{p}"
+        lines.append(f"@{nxt} = r & @mul ~ (@{cur} (@{have[p]} r))")
+        cur = nxt
+    return lines, cur
+
+
+def church_loop(n: int = 12, m: int = 12) -> Tuple[str, int]:
+    """Outer loop of 2^n iterations: iteration i applies the Church numeral
+    2^m to the increment starting at i; the loop sums the results."""
+    lines, cur = church_numeral_defs(m)
+    lines += ["@loop = (?((@loop_z @loop_s) (a r)) (a r))",
+              "@loop_z = (a a)",
+              f"@loop_s = ({{i1 i2}} (a r)) & @{cur} ~ (@inc (i1 v)) & a ~ $([+] $(v a2)) & @loop ~ (i2 (a2 r))",
+              f"@main = r & @loop ~ ({1 << n} (0 r))"]
+    N = 1 << n
+    return "\n".join(lines) + "\n", (N * (N - 1) // 2 + N * (1 << m)) % U24
+
+
 def church(n: int = 12, m: int = 12) -> Tuple[str, int]:
     """2^n iterations of (Church 2^m) applied to x+1 from 0, summed."""
     assert m >= 1
@@ -374,4 +501,6 @@ CONFIGS = {
     "bitonic_2e16": lambda: bitonic(16, 0),
     "radix_2e18": lambda: radix(18, 0),
     "church_2e12x2e12": lambda: church(12, 12),
+    "radix_lsd_2e18": lambda: radix_lsd(18, 0),
+    "church_loop_2e12x2e12": lambda: church_loop(12, 12),
 }

@@ -136,3 +136,89 @@ def run_reference(words, threads: int = 8, timeout: float = 600.0) -> RefResult:
         if m:
             mips = float(m.group(1))
     return RefResult(p.returncode, res, itrs, t, mips, p.stdout, p.stderr)
+
+
+# ---------------------------------------------------------------------------
+# Protocol model (hvm_model.cpp): the HIP turn kernel's steps executed on the
+# CPU with the memory operations of all waves interleaved in a seeded random
+# order.  It counts the events the protocol rules out (held vars read as free,
+# takes that find no node, links that exceed the round cap, put-backs over a
+# new node) and returns ITRS and the normal form for comparison with the
+# oracle.  TEST INFRASTRUCTURE ONLY, like everything in this directory.
+# ---------------------------------------------------------------------------
+
+MODEL_LIB = os.path.join(HERE, "libhvm_model.so")
+MODEL_XCHG_TWO = 1
+MODEL_LATE_TAKES = 2
+MODEL_OVERLAP_SCAN = 4
+
+
+class _ModelCfg(C.Structure):
+    _fields_ = [("waves", C.c_uint32), ("lanes", C.c_uint32), ("deq", C.c_uint32), ("tbl", C.c_uint32),
+                ("node_cap", C.c_uint32), ("vars_cap", C.c_uint32), ("root_vars", C.c_uint32),
+                ("bag_cap", C.c_uint32), ("turn_steps", C.c_uint32), ("idle_pct", C.c_uint32),
+                ("seed", C.c_uint32), ("switch_pct", C.c_uint32), ("variant", C.c_uint32),
+                ("link_cap", C.c_uint32), ("max_ops", C.c_uint64)]
+
+
+class _ModelOut(C.Structure):
+    _fields_ = [("itrs", C.c_uint64), ("kind", C.c_uint64 * 10), ("err", C.c_uint32), ("turns", C.c_uint32),
+                ("zero_publish", C.c_uint64), ("zero_enter", C.c_uint64), ("take_zero", C.c_uint64),
+                ("putback", C.c_uint64), ("clobber", C.c_uint64), ("double_alloc", C.c_uint64),
+                ("max_rounds", C.c_uint64), ("full_scans", C.c_uint64), ("pool_claims", C.c_uint64),
+                ("ops", C.c_uint64), ("first_event_op", C.c_uint64),
+                ("first_event", C.c_uint32), ("first_err_turn", C.c_uint32), ("first_err_op", C.c_uint64),
+                ("result", C.c_void_p)]
+
+
+@dataclass
+class ModelResult:
+    status: int
+    itrs: int
+    kinds: dict
+    err: int
+    turns: int
+    events: dict
+    results: list
+
+
+_mlib = None
+
+
+def model_lib():
+    global _mlib
+    if _mlib is None:
+        if not os.path.exists(MODEL_LIB):
+            raise RuntimeError(f"{MODEL_LIB} missing: run `make -C oracle`")
+        L = C.CDLL(MODEL_LIB)
+        L.hvm_model_run.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32), C.c_uint32,
+                                    C.POINTER(_ModelCfg), C.POINTER(_ModelOut)]
+        L.hvm_model_run.restype = C.c_int
+        L.hvm_model_free.argtypes = [C.c_void_p]
+        _mlib = L
+    return _mlib
+
+
+def run_model(words, entries=(0,), **cfg) -> ModelResult:
+    """Runs the protocol model on a book; `entries` are the fids of the nets
+    booted together (one: @main against ROOT; several: one root var each)."""
+    L = model_lib()
+    c = _ModelCfg()
+    defaults = dict(waves=4, lanes=8, deq=8, tbl=256, node_cap=1 << 14, vars_cap=1 << 14, root_vars=0,
+                    bag_cap=0, turn_steps=0, idle_pct=0, seed=0, switch_pct=100, variant=0, link_cap=0,
+                    max_ops=0)
+    defaults.update(cfg)
+    for k, v in defaults.items():
+        setattr(c, k, v)
+    arr = (C.c_uint32 * len(words))(*words)
+    ent = (C.c_uint32 * len(entries))(*entries)
+    o = _ModelOut()
+    rc = L.hvm_model_run(arr, len(words), ent, len(entries), C.byref(c), C.byref(o))
+    res = []
+    if o.result:
+        res = C.string_at(o.result).decode().split(";")
+        L.hvm_model_free(o.result)
+    ev = {k: getattr(o, k) for k in ("zero_publish", "zero_enter", "take_zero", "putback", "clobber",
+                                     "double_alloc", "max_rounds", "full_scans", "pool_claims", "ops", "first_event_op", "first_event",
+                                     "first_err_op")}
+    return ModelResult(rc, o.itrs, {k: o.kind[i] for i, k in enumerate(KINDS)}, o.err, o.turns, ev, res)

@@ -1,7 +1,9 @@
 """Writes tests/golden/golden.json: expected normal forms and interaction counts.
 
 Run in the build container (needs /root/reference and oracle/_ref built):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                  # everything
+    python tests/golden/make_golden.py --only radix_lsd,church_loop
+                                  # re-record the workloads of these generators only
 
 Each case records the program text (the reference's own test programs and
 examples are its test data; the synthetic workloads come from
@@ -60,7 +62,38 @@ def record(name, code, extra=None, threads=8, timeout=3000):
     return case
 
 
+SMALL = ([("sum_tree", {"depth": d}) for d in (1, 4, 10, 16)]
+         + [("bitonic", {"k": k, "seed": s}) for k, s in ((1, 0), (3, 1), (6, 2), (8, 3), (10, 4), (12, 5))]
+         + [("radix", {"k": k, "seed": s}) for k, s in ((1, 0), (4, 1), (8, 2), (10, 3), (12, 4))]
+         + [("church", {"n": n, "m": m}) for n, m in ((1, 2), (3, 4), (6, 8), (8, 12))]
+         + [("radix_lsd", {"k": k, "seed": s}) for k, s in ((1, 0), (4, 1), (8, 2), (10, 3), (12, 4))]
+         + [("radix_lsd", {"k": 6, "seed": 5, "bits": 3})]
+         + [("church_loop", {"n": n, "m": m}) for n, m in ((1, 2), (3, 4), (6, 8), (8, 12))])
+FULL = [("sum_tree", {"depth": 20}), ("bitonic", {"k": 16, "seed": 0}),
+        ("radix", {"k": 18, "seed": 0}), ("church", {"n": 12, "m": 12}),
+        ("radix_lsd", {"k": 18, "seed": 0}), ("church_loop", {"n": 12, "m": 12})]
+
+
+def workload_case(gen, params):
+    code, expect = getattr(P, gen)(**params)
+    name = gen + "_" + "_".join(f"{k}{v}" for k, v in params.items())
+    case = record(name, code, {"generator": gen, "params": params, "full_size": (gen, params) in FULL})
+    case["closed_form"] = "sorted keys" if gen == "bitonic" else str(expect)
+    return case
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--only":
+        gens = set(sys.argv[2].split(","))
+        path = os.path.join(ROOT, "tests", "golden", "golden.json")
+        out = json.load(open(path))
+        out["workloads"] = [c for c in out["workloads"] if c["generator"] not in gens]
+        for gen, params in SMALL + FULL:
+            if gen in gens:
+                out["workloads"].append(workload_case(gen, params))
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        return
     out = {"reference": "compiled reference run-c (src/run.c + src/hvm.c, -O3 -DIO -DTPC_L2=3)",
            "programs": [], "workloads": []}
     files = sorted(glob.glob(os.path.join(REF, "examples", "*", "*.hvm")) +
@@ -101,22 +134,8 @@ def main():
     for name, code in edges.items():
         out["programs"].append(record(name, code, {"code": code, "snapshot": None}))
 
-    small = [("sum_tree", {"depth": d}) for d in (1, 4, 10, 16)]
-    small += [("bitonic", {"k": k, "seed": s}) for k, s in ((1, 0), (3, 1), (6, 2), (8, 3), (10, 4), (12, 5))]
-    small += [("radix", {"k": k, "seed": s}) for k, s in ((1, 0), (4, 1), (8, 2), (10, 3), (12, 4))]
-    small += [("church", {"n": n, "m": m}) for n, m in ((1, 2), (3, 4), (6, 8), (8, 12))]
-    full = [("sum_tree", {"depth": 20}), ("bitonic", {"k": 16, "seed": 0}),
-            ("radix", {"k": 18, "seed": 0}), ("church", {"n": 12, "m": 12})]
-    for gen, params in small + full:
-        code, expect = getattr(P, gen)(**params)
-        name = gen + "_" + "_".join(f"{k}{v}" for k, v in params.items())
-        case = record(name, code, {"generator": gen, "params": params,
-                                   "full_size": (gen, params) in full})
-        if gen == "bitonic":
-            case["closed_form"] = "sorted keys"
-        else:
-            case["closed_form"] = str(expect)
-        out["workloads"].append(case)
+    for gen, params in SMALL + FULL:
+        out["workloads"].append(workload_case(gen, params))
     with open(os.path.join(ROOT, "tests", "golden", "golden.json"), "w") as f:
         json.dump(out, f, indent=1)
 

@@ -88,14 +88,23 @@ def _tuned_batch_cfg():
     return json.load(open(path))["config"] if os.path.exists(path) else None
 
 
-@pytest.mark.parametrize("which", ["default", "tuned"])
+@pytest.mark.parametrize("which", ["default", "tuned", "idle_off", "idle_off_512"])
 def test_batch_matches_per_net_reference(hip, which):
+    """idle_off: turns never end early (idle_pct above 100), so few busy waves
+    run long turns, fill their slices and allocate from the shared pool; the
+    case that raised the link-loop flag before the allocator's scan stopped
+    revisiting its first window (tests/test_model.py holds the CPU model of
+    it).  idle_off_512: the same on a quarter of the default wave count."""
     g = json.load(open(os.path.join(ROOT, "tests", "golden", "batch.json")))
     code, nets = P.batch(g["seed"], g["n"])
     fids = P.def_fids(code)
     entries = [fids[f"net{i}"] for i in range(len(nets))]
     est = [P.batch_estimate(gen, params) for gen, params, _, _ in nets]
     cfg = {"node_cap": 1 << 29, "vars_cap": 1 << 29}
+    if which.startswith("idle_off"):
+        cfg["idle_pct"] = 101
+        if which.endswith("512"):
+            cfg["waves"] = 512
     if which == "tuned":
         if _tuned_batch_cfg() is None:
             pytest.skip("no tuned configuration committed")
@@ -150,3 +159,5 @@ def test_autotuner_keeps_results(hip):
     assert rep["itrs"] > 0
     assert rep["tuned_ms"] <= rep["default_ms"]
     assert any(m.get("cfg") == rep["config"] for m in rep["measurements"])
+    # evaluator errors are raised, never recorded as a skipped candidate
+    assert not any("flags" in m.get("error", "") for m in rep["measurements"])

@@ -1,0 +1,108 @@
+"""The HIP turn kernel's protocol, run on the CPU by the protocol model
+(oracle/hvm_model.cpp): the kernel's steps with the memory operations of all
+waves interleaved in a seeded random order.
+
+The batched workload with idle quit off is the case that raised the kernel's
+link-loop flag (0x8) on the GPU: few busy waves run long turns, fill their
+node and var slices and allocate from the shared pool.  Its cause, found with
+this model: an allocation that scanned its whole slice went on for one more
+window, wrapped around onto the slots it had handed out earlier in the same
+call (still zero until the creation phase) and handed them out again.  Two
+interactions then shared a var slot, a held var read as free, and a link
+published into slot 0 (ROOT's slot, zero in a batched evaluation) forever.
+
+Checks: with the kernel's current protocol no such event occurs on any seed
+and every net's normal form and the total ITRS equal the oracle's; with the
+old scan planted back into the model, the events and the flag come back (so
+the model does see them).  The batch is a small replica of the 256-net
+workload (sum trees and bitonic sorts, one root var each).
+"""
+import pytest
+
+import programs as P
+
+ANOMALIES = ("zero_publish", "zero_enter", "take_zero", "putback", "clobber")
+
+
+def mini_batch(seed, n, dmin=4, dmax=9, kmin=2, kmax=5):
+    g = P.splitmix64(seed ^ 0x5EED_BA7C)
+    body = ["@main = *", P.SUM_DEFS.strip(), P.BITONIC_DEFS.strip()]
+    nets = []
+    for i in range(n):
+        if i < n // 2:
+            d = dmin + next(g) % (dmax - dmin + 1)
+            body.append(f"@net{i} = r & @sum ~ ({d} r)")
+            nets.append((P.sum_tree(d)[0], str((1 << d) % (1 << 24))))
+        else:
+            k = kmin + next(g) % (kmax - kmin + 1)
+            ks = next(g) & 0xFFFFFFFF
+            keys = P.keys_uniform(1 << k, ks)
+            top, lines = P.key_tree_defs(keys, f"k{i}", 3)
+            body.append(f"@net{i} = r & @sort ~ ({k} (0 (@{top} r)))")
+            body.extend(lines)
+            nets.append((P.bitonic(k, ks, 3)[0], sorted(keys)))
+    code = "\n".join(body) + "\n"
+    fids = P.def_fids(code)
+    return code, [fids[f"net{i}"] for i in range(n)], nets
+
+
+def result_ok(text, expect):
+    if isinstance(expect, list):
+        try:
+            return P.flatten_tree(P.parse_tree_text(text)) == expect
+        except Exception:
+            return False
+    return text == expect
+
+
+# 16 waves of 8 lanes, 16K-slot net: slices fill, the pool is used
+CFG = dict(waves=16, lanes=8, deq=8, node_cap=1 << 14, vars_cap=1 << 14, root_vars=4097, idle_pct=101,
+           max_ops=20_000_000)
+
+
+@pytest.fixture(scope="module")
+def batch(hip, orc):
+    code, entries, nets = mini_batch(0, 32)
+    words = hip.compile_book(code)
+    itrs = sum(orc.run_oracle(hip.compile_book(c)).itrs for c, _ in nets)
+    return words, entries, nets, itrs
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_batch_idle_off_is_clean(batch, orc, seed):
+    words, entries, nets, itrs = batch
+    r = orc.run_model(words, entries, seed=seed, **CFG)
+    assert r.status == 0 and r.err == 0
+    assert {k: r.events[k] for k in ANOMALIES} == {k: 0 for k in ANOMALIES}
+    assert r.events["full_scans"] > 0 and r.events["pool_claims"] > 0   # the case is exercised
+    assert r.itrs == itrs
+    assert all(result_ok(t, e) for t, (_, e) in zip(r.results, nets))
+
+
+def test_batch_idle_off_exchange_takes_is_clean(batch, orc):
+    # the earlier take mode of ANNI/COMM (exchanges, put back on a retry)
+    words, entries, nets, itrs = batch
+    r = orc.run_model(words, entries, seed=3, variant=orc.MODEL_XCHG_TWO, **CFG)
+    assert r.err == 0 and r.itrs == itrs
+    assert r.events["take_zero"] == 0
+
+
+def test_overlapping_scan_is_detected(batch, orc):
+    # the old allocator scan (len + one window) planted back into the model
+    words, entries, nets, itrs = batch
+    r = orc.run_model(words, entries, seed=0, variant=orc.MODEL_OVERLAP_SCAN, **CFG)
+    assert r.err & 0x8                       # HVM_ERR_LINK_LOOP, as on the GPU
+    assert r.events["zero_publish"] + r.events["zero_enter"] > 0
+    assert r.events["first_event_op"] < r.events["first_err_op"]   # the anomaly comes first
+
+
+@pytest.mark.parametrize("gen,args", [("sum_tree", (8,)), ("bitonic", (5, 1)), ("radix", (4, 2)),
+                                      ("church", (3, 4))])
+def test_single_nets_match_oracle(hip, orc, gen, args):
+    code, _ = getattr(P, gen)(*args)
+    w = hip.compile_book(code)
+    o = orc.run_oracle(w)
+    for seed in range(2):
+        r = orc.run_model(w, seed=seed, waves=4, lanes=8, node_cap=1 << 15, vars_cap=1 << 15, turn_steps=5)
+        assert r.err == 0 and r.itrs == o.itrs and r.kinds == o.kinds
+        assert hip.canonical(r.results[0]) == hip.canonical(o.result)
