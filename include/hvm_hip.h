@@ -43,7 +43,9 @@ typedef struct hvm_hip_cfg {
   uint64_t node_cap;    /* node slots in the net (0: 2^28; at most 2^29, or
                            2^32 - 2 with HVM_HIP_WIDE)                        */
   uint64_t vars_cap;    /* var slots in the net (0: 2^28; limits as node_cap) */
-  uint32_t bag_cap;     /* redex-bag entries per bag (0: 2^24)                */
+  uint32_t bag_cap;     /* redex-bag entries per bag (0: node_cap + 4097, as
+                           many as the net can hold redexes, so the bags
+                           never overflow; 2^24 with HVM_HIP_WIDE)           */
   uint32_t waves;       /* evaluator waves (0: fill every CU)                 */
   uint32_t turn_steps;  /* steps per turn (0: adaptive; at most 2^20)         */
   uint32_t flags;       /* HVM_HIP_NO_READBACK: skip the result text          */

@@ -37,6 +37,16 @@ typedef uint32_t u32;
 
 __device__ __forceinline__ u32 vidx(u32 val) { return val == HVM_ROOT_VAL ? 0u : val; }
 
+// Per-turn hash of the scheduler seed (placement rotation, turn-length jitter
+// and exploration draws).
+__host__ __device__ __forceinline__ u32 sched_hash(u32 seed, u32 turn) {
+  u32 h = (seed * 0x9E3779B9u) ^ (turn * 0x85EBCA6Bu);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
+
 // Every access is bounds-checked against the net's capacity: a port that
 // points outside the net reaches the spare slot at index cap (each buffer has
 // one past its capacity), never memory the net does not own.  The check is a
@@ -549,10 +559,15 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   if (ws.vars_ovf < A.vars_pool_lo || ws.vars_ovf >= A.vars_pool_hi)
     ws.vars_ovf = A.vars_pool_lo + (u32)(((u64)(A.vars_pool_hi - A.vars_pool_lo) * w) / A.nwaves);
 
-  // Initial spread: lane l of wave w takes item w + l*W, so a small bag is
-  // spread one redex per wave before any wave takes a second one.
+  // Initial spread: lane l of wave w takes item w' + l*W, so a small bag is
+  // spread one redex per wave before any wave takes a second one.  A nonzero
+  // scheduler seed rotates the waves (w' = w + a per-turn offset), so that
+  // seeds give different placements, hence different schedules; results do
+  // not depend on the placement.
   {
-    u32 idx = w + lane * A.nwaves;
+    const u32 seed = A.sched->seed;
+    const u32 rot = seed ? sched_hash(seed, C->turns) % A.nwaves : 0u;
+    u32 idx = (w + rot) % A.nwaves + lane * A.nwaves;
     if (idx < n_in && idx < HVM_WAVE * A.nwaves) push_redex<LOC>(L, N, ln, A, bag_in[idx]);
   }
   bool more = n_in > HVM_WAVE * A.nwaves;
@@ -689,7 +704,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     // has nothing to put back (src/hvm.c:921-924 checks both nodes before
     // taking either for the same reason).
     const bool PEEK = EARLY && (HVM_PEEK_TAKES || take_a0);
-    auto hbm_take = [PEEK](const HvmTurnArgs& A, u32 loc) { return PEEK ? node_peek(A, loc) : node_take(A, loc); };
+    auto hbm_take = [](const HvmTurnArgs& A, u32 loc, bool peek) { return peek ? node_peek(A, loc) : node_take(A, loc); };
     // slots from the net's main allocator (local with LOC), known here for
     // every rule but CALL
     u32 nn = 0, nv = 0;
@@ -717,8 +732,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       const u32 o = wave_scan(rule != HVM_CALL ? (nn | (nv << 16)) : 0u, &tot);
       off_n = o & 0xFFFFu; off_v = o >> 16;
       run = has && rule != HVM_CALL && off_n + nn <= HVM_TBL_NODE && off_v + nv <= HVM_TBL_VARS;
-      NA = (run && take_a0) ? hbm_take(A, hvm_get_val(a)) : 0ull;
-      NB = (run && take_b0) ? hbm_take(A, hvm_get_val(b)) : 0ull;
+      NA = (run && take_a0) ? hbm_take(A, hvm_get_val(a), PEEK) : 0ull;
+      NB = (run && take_b0) ? hbm_take(A, hvm_get_val(b), PEEK) : 0ull;
     }
     // A CALL's def record with its first template pairs (one 128-byte load).
     bool copy = false;
@@ -807,7 +822,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       NA = take_a ? net_take<LOC>(A, N, hvm_get_val(a)) : 0ull;
       NB = take_b ? net_take<LOC>(A, N, hvm_get_val(b)) : 0ull;
     } else if (run && rule == HVM_CALL && (copy || fastb)) {
-      NB = hbm_take(A, hvm_get_val(b));
+      NB = hbm_take(A, hvm_get_val(b), PEEK);
     }
     const u64* tpre = dx.node;
     const u64* rpre = dx.rbag;
@@ -1475,12 +1490,14 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
     for (u32 k = 1; k < HVM_ACTIONS; ++k) if (y[k] > y[best]) best = k;
     if (y[best] > y[action] + 0.01f * (float)s->explore_pct) action = best;
   } else if (s->mode == HVM_SCHED_EXPLORE) {
-    u32 h = (s->seed * 0x9E3779B9u) ^ (c->turns * 0x85EBCA6Bu);
-    h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12;
+    const u32 h = sched_hash(s->seed, c->turns);
     if (h % 100u < s->explore_pct) action = (h >> 8) & 3u;
   }
   if (t == 0) {
-    c->max_steps = c->fixed_steps ? c->fixed_steps : kTurnLen[action];
+    // a nonzero seed shortens each turn by up to an eighth (share jitter)
+    const u32 len = kTurnLen[action];
+    const u32 jit = s->seed ? (sched_hash(s->seed ^ 0xA5A5A5A5u, c->turns) >> 4) % (len / 8u + 1u) : 0u;
+    c->max_steps = c->fixed_steps ? c->fixed_steps : len - jit;
     c->idle_quit = kIdlePct[action] ? max(1u, (A.nwaves * kIdlePct[action]) / 100u) : A.idle_quit;
     if (s_steady) c->idle_quit = 0;
     s->last_action = action;

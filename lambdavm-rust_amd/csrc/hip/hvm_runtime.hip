@@ -118,7 +118,10 @@ struct hvm_hip_ctx {
       return fail("node_cap/vars_cap exceed 2^28, the limit of the local-net evaluator");
     node_cap = (u32)ncap;
     vars_cap = (u32)vcap;
-    bag_cap = cfg.bag_cap ? cfg.bag_cap : (1u << 24);
+    // Every redex the evaluator pushes holds the principal port of at least
+    // one live node (VOID redexes are counted where they are created), so a
+    // bag of node_cap entries plus one per batched root never overflows.
+    bag_cap = cfg.bag_cap ? cfg.bag_cap : node_cap + 1 + HVM_HIP_MAX_NETS;
     if (cfg.waves) {
       nwaves = cfg.waves;
     } else {

@@ -71,6 +71,7 @@ typedef struct hvm_model_out {
   uint64_t max_rounds;     // longest link batch, in rounds
   uint64_t full_scans;     // allocations that looked at every slot of the wave's slice
   uint64_t pool_claims;    // slots claimed in the shared pool
+  uint64_t defers;         // lane-steps that waited for allocation-table space
   uint64_t ops;            // net operations performed
   uint64_t first_event_op; // op index of the first anomaly (0: none)
   uint32_t first_event;    // its kind: 1 zero publish, 2 zero enter, 3 take zero, 4 link cap, 5 clobber
@@ -83,6 +84,9 @@ typedef struct hvm_model_out {
 #define HVM_MODEL_XCHG_TWO 1u    // two-take rules (ANNI, COMM) take by exchange (else by load, zeroed at commit)
 #define HVM_MODEL_LATE_TAKES 2u  // every take after the step's allocation (no early takes)
 #define HVM_MODEL_OVERLAP_SCAN 4u // the allocator's scan covers len + one window (revisits its first window)
+#define HVM_MODEL_TAKE_BEFORE_FIT 8u // every lane with a take rule takes (by exchange) before knowing
+                                  // whether its step fits the tables; lanes that then wait put
+                                  // their nodes back
 
 int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_t* entries, uint32_t n_nets,
                   const hvm_model_cfg* cfg, hvm_model_out* out);
@@ -530,7 +534,8 @@ void Model::turn(u32 w) {
       on += x.has ? nn : 0; ov += x.has ? nv : 0;
       x.run = x.has && x.rule != HVM_CALL && x.off_n + x.nn <= TB && x.off_v + x.nv <= TB;
     }
-    const bool peek_two = (C.variant & HVM_MODEL_XCHG_TWO) == 0;
+    const bool before_fit = (C.variant & HVM_MODEL_TAKE_BEFORE_FIT) != 0;
+    const bool peek_two = (C.variant & HVM_MODEL_XCHG_TWO) == 0 && !before_fit;
     auto two = [&](const St& x) { return x.take_a && x.take_b; };
     // the round trip of the early takes and the allocator's first windows
     std::vector<u64> pre_n(L, 1);
@@ -538,8 +543,9 @@ void Model::turn(u32 w) {
     {
       std::vector<u32> ops;   // 0..L-1: take a, L..2L-1: take b, 2L..3L-1: node window, 3L..: var window
       for (u32 l = 0; l < L; ++l) {
-        if (early && st[l].run && st[l].take_a) ops.push_back(l);
-        if (early && st[l].run && st[l].take_b) ops.push_back(L + l);
+        const bool tk = early && (st[l].run || (before_fit && st[l].has && st[l].rule != HVM_CALL));
+        if (tk && st[l].take_a) ops.push_back(l);
+        if (tk && st[l].take_b) ops.push_back(L + l);
         ops.push_back(2 * L + l);
         ops.push_back(3 * L + l);
       }
@@ -582,7 +588,17 @@ void Model::turn(u32 w) {
       St& x = st[l];
       if (x.run) { need_n = std::max(need_n, x.off_n + x.nn); need_v = std::max(need_v, x.off_v + x.nv); }
       if (x.run) lanes[l].top--;
+      if (x.has && !x.run) O->defers++;
     }
+    // lanes that took before knowing that they fit, and wait: nodes go back
+    if (before_fit)
+      for (u32 l : order(L)) {
+        St& x = st[l];
+        if (x.run || !x.has || x.rule == HVM_CALL) continue;
+        if (x.take_a && x.NA) { O->putback++; if (node_gen[nslot(hvm_get_val(x.a))] != x.ga) { O->clobber++; event(5); } node_store(hvm_get_val(x.a), x.NA); }
+        if (x.take_b && x.NB) { O->putback++; if (node_gen[nslot(hvm_get_val(x.b))] != x.gb) { O->clobber++; event(5); } node_store(hvm_get_val(x.b), x.NB); }
+        x.NA = x.NB = 0;
+      }
     // ---- late takes: a safe def's copy (all takes without early takes) ----
     {
       std::vector<u32> ops;

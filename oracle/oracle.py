@@ -151,6 +151,7 @@ MODEL_LIB = os.path.join(HERE, "libhvm_model.so")
 MODEL_XCHG_TWO = 1
 MODEL_LATE_TAKES = 2
 MODEL_OVERLAP_SCAN = 4
+MODEL_TAKE_BEFORE_FIT = 8
 
 
 class _ModelCfg(C.Structure):
@@ -165,7 +166,7 @@ class _ModelOut(C.Structure):
     _fields_ = [("itrs", C.c_uint64), ("kind", C.c_uint64 * 10), ("err", C.c_uint32), ("turns", C.c_uint32),
                 ("zero_publish", C.c_uint64), ("zero_enter", C.c_uint64), ("take_zero", C.c_uint64),
                 ("putback", C.c_uint64), ("clobber", C.c_uint64), ("double_alloc", C.c_uint64),
-                ("max_rounds", C.c_uint64), ("full_scans", C.c_uint64), ("pool_claims", C.c_uint64),
+                ("max_rounds", C.c_uint64), ("full_scans", C.c_uint64), ("pool_claims", C.c_uint64), ("defers", C.c_uint64),
                 ("ops", C.c_uint64), ("first_event_op", C.c_uint64),
                 ("first_event", C.c_uint32), ("first_err_turn", C.c_uint32), ("first_err_op", C.c_uint64),
                 ("result", C.c_void_p)]
@@ -219,6 +220,6 @@ def run_model(words, entries=(0,), **cfg) -> ModelResult:
         res = C.string_at(o.result).decode().split(";")
         L.hvm_model_free(o.result)
     ev = {k: getattr(o, k) for k in ("zero_publish", "zero_enter", "take_zero", "putback", "clobber",
-                                     "double_alloc", "max_rounds", "full_scans", "pool_claims", "ops", "first_event_op", "first_event",
+                                     "double_alloc", "max_rounds", "full_scans", "pool_claims", "defers", "ops", "first_event_op", "first_event",
                                      "first_err_op")}
     return ModelResult(rc, o.itrs, {k: o.kind[i] for i, k in enumerate(KINDS)}, o.err, o.turns, ev, res)

@@ -71,7 +71,8 @@ SMALL = ([("sum_tree", {"depth": d}) for d in (1, 4, 10, 16)]
          + [("church_loop", {"n": n, "m": m}) for n, m in ((1, 2), (3, 4), (6, 8), (8, 12))])
 FULL = [("sum_tree", {"depth": 20}), ("bitonic", {"k": 16, "seed": 0}),
         ("radix", {"k": 18, "seed": 0}), ("church", {"n": 12, "m": 12}),
-        ("radix_lsd", {"k": 18, "seed": 0}), ("church_loop", {"n": 12, "m": 12})]
+        ("radix_lsd", {"k": 18, "seed": 0}), ("church_loop", {"n": 12, "m": 12}),
+        ("stress", {"depth": 18, "length": 1024})]
 
 
 def workload_case(gen, params):

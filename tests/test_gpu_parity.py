@@ -62,7 +62,10 @@ def test_small_workloads(golden, hip, ctx):
             assert r.result == str(expect)
 
 
-@pytest.mark.parametrize("gen", ["sum_tree", "bitonic", "radix", "church"])
+# stress: the reference's stress program with 2^18 loops of 1024 iterations,
+# whose frontier (19 M redexes) overflowed the fixed 2^24-entry bags; bags
+# are now sized from the node capacity
+@pytest.mark.parametrize("gen", ["sum_tree", "bitonic", "radix", "church", "radix_lsd", "church_loop", "stress"])
 def test_full_size_workloads(golden, hip, ctx, gen):
     case = next(c for c in golden["workloads"] if c.get("full_size") and c["generator"] == gen)
     print("case", case["name"], flush=True)
@@ -79,7 +82,8 @@ def test_full_size_workloads(golden, hip, ctx, gen):
 
 
 def test_histogram_matches_oracle(hip, ctx, orc):
-    for code in [P.sum_tree(10)[0], P.bitonic(6, 2)[0], P.radix(6, 1)[0], P.church(3, 4)[0]]:
+    for code in [P.sum_tree(10)[0], P.bitonic(6, 2)[0], P.radix(6, 1)[0], P.church(3, 4)[0],
+                 P.radix_lsd(6, 3)[0], P.church_loop(4, 6)[0]]:
         w = hip.compile_book(code)
         o = orc.run_oracle(w)
         ctx.load(w)

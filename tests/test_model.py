@@ -106,3 +106,23 @@ def test_single_nets_match_oracle(hip, orc, gen, args):
         r = orc.run_model(w, seed=seed, waves=4, lanes=8, node_cap=1 << 15, vars_cap=1 << 15, turn_steps=5)
         assert r.err == 0 and r.itrs == o.itrs and r.kinds == o.kinds
         assert hip.canonical(r.results[0]) == hip.canonical(o.result)
+
+
+def test_takes_only_after_the_step_fits(hip, orc):
+    """A lane takes its nodes only once its step is known to fit the
+    allocation tables.  Planted in the model: every lane takes first and a lane
+    that then has to wait puts its nodes back (the local-net evaluator's
+    early-take variant did this); meanwhile the owning wave's allocator can
+    hand out a slot that reads zero, and the put-back overwrites the new node.
+    The model sees those overwrites, and wrong results follow; the kernel's
+    protocol has none on the same schedule.  A COMM-heavy program on 64-lane
+    waves with 16-entry tables makes lanes wait often."""
+    code, expect = P.church_loop(6, 8)
+    w = hip.compile_book(code)
+    cfg = dict(waves=4, lanes=64, deq=16, tbl=16, node_cap=1 << 13, vars_cap=1 << 13, max_ops=30_000_000)
+    ok = orc.run_model(w, seed=0, **cfg)
+    assert ok.err == 0 and ok.results == [str(expect)] and ok.events["defers"] > 0
+    assert {k: ok.events[k] for k in ANOMALIES} == {k: 0 for k in ANOMALIES}
+    bad = orc.run_model(w, seed=0, variant=orc.MODEL_TAKE_BEFORE_FIT, **cfg)
+    assert bad.events["clobber"] > 0
+    assert bad.results != [str(expect)]
