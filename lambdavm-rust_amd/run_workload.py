@@ -3,7 +3,8 @@
     python lambdavm-rust_amd/run_workload.py stress --args 8,4096 [--mode compiled]
 
 Workloads are the generators of programs.py (sum_tree, bitonic, radix,
-church, stress, tree_gen) with their parameters in order; modes are the
+radix_lsd, church, church_loop, stress, tree_gen) with their parameters in
+order; modes are the
 evaluators of include/hvm_hip.h (interpreted, compiled CALLs, local nets,
 64-bit ports).  Prints one JSON line: ITRS, evaluation time, MIPS, wave-steps,
 turns, spills, and the turn kernel's HBM roofline fraction: the algorithmic
@@ -28,7 +29,8 @@ MODES = {"interpreted": 0, "compiled": H.COMPILED_CALLS, "local": H.LOCAL_NET, "
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=("sum_tree", "bitonic", "radix", "church", "stress", "tree_gen"))
+    ap.add_argument("workload", choices=("sum_tree", "bitonic", "radix", "radix_lsd", "church", "church_loop",
+                                         "stress", "tree_gen"))
     ap.add_argument("--args", default="", help="comma-separated generator parameters")
     ap.add_argument("--mode", choices=tuple(MODES), default="interpreted")
     ap.add_argument("--time-limit", type=int, default=60)

@@ -69,7 +69,7 @@ typedef struct hvm_model_out {
   uint64_t clobber;        // a put-back overwrote a node created after the take
   uint64_t double_alloc;   // a slot handed out while still in use
   uint64_t max_rounds;     // longest link batch, in rounds
-  uint64_t full_scans;     // allocations that looked at every slot of the wave's slice
+  uint64_t full_scans;     // allocations that moved on from the wave's slice to the shared pool
   uint64_t pool_claims;    // slots claimed in the shared pool
   uint64_t defers;         // lane-steps that waited for allocation-table space
   uint64_t ops;            // net operations performed
@@ -336,12 +336,13 @@ void Model::link_lanes(std::vector<std::vector<std::pair<u32, u32>>>& batches, s
 // by compare-and-swap).
 bool Model::wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::vector<u32>& tn, std::vector<u32>& tv,
                        bool pre, const std::vector<u64>& pre_n, const std::vector<u32>& pre_v, u32& oom) {
-  struct Scan { u32 lo, len, cur, got, need, scanned; bool pool; u32 cursor; };
+  struct Scan { u32 lo, len, cur, got, need, scanned; bool pool; u32 cursor, slice_next; };
   const u32 L = C.lanes;
   const bool fix_scan = !(C.variant & HVM_MODEL_OVERLAP_SCAN);
+  const u32 slice_windows = (C.variant & HVM_MODEL_OVERLAP_SCAN) ? 0xFFFFFFu : 8u;
   WaveState& cs = ws[w];
-  Scan sn = {n_lo, node_slice, cs.node_cur, 0, need_n, 0, false, cs.node_cur};
-  Scan sv = {v_lo, vars_slice, cs.vars_cur, 0, need_v, 0, false, cs.vars_cur};
+  Scan sn = {n_lo, node_slice, cs.node_cur, 0, need_n, 0, false, cs.node_cur, cs.node_cur};
+  Scan sv = {v_lo, vars_slice, cs.vars_cur, 0, need_v, 0, false, cs.vars_cur, cs.vars_cur};
   auto idx = [&](Scan& s, u32 lane, u32& rel) { rel = (s.cur - s.lo + lane) % s.len; return s.lo + rel; };
   auto window = [&](auto& buf, Scan& s, std::vector<u32>& tbl, u32 reserved, auto claim, auto* vals) {
     std::vector<u32> slot(L), rel(L);
@@ -379,9 +380,13 @@ bool Model::wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::v
     }
   };
   auto advance = [&](Scan& s, u32 pool_lo, u32 pool_len, u32 pool_cur) {
-    if (s.got >= s.need || s.scanned < (fix_scan ? s.len : s.len + L)) return true;
+    // the slice: at most slice_windows windows per allocation (the kernel's
+    // HVM_SLICE_SCAN), continued from there by the next one
+    const u32 lim = s.pool ? s.len : std::min(s.len, slice_windows * L);
+    if (s.got >= s.need || s.scanned < (fix_scan ? lim : s.len + L)) return true;
     if (s.pool || pool_len < L) return false;
     O->full_scans++;
+    s.slice_next = s.cur;
     s.pool = true;
     s.lo = pool_lo;
     s.len = pool_len;
@@ -414,8 +419,8 @@ bool Model::wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::v
       if (!advance(sv, vars_pool_lo, vars_pool_hi - vars_pool_lo, cs.vars_ovf)) { oom = ERR_OOM_VARS; return false; }
     }
   }
-  if (need_n) { if (sn.pool) cs.node_ovf = sn.cursor; else cs.node_cur = sn.cursor; }
-  if (need_v) { if (sv.pool) cs.vars_ovf = sv.cursor; else cs.vars_cur = sv.cursor; }
+  if (need_n) { if (sn.pool) { cs.node_ovf = sn.cursor; cs.node_cur = sn.slice_next; } else cs.node_cur = sn.cursor; }
+  if (need_v) { if (sv.pool) { cs.vars_ovf = sv.cursor; cs.vars_cur = sv.slice_next; } else cs.vars_cur = sv.cursor; }
   return true;
 }
 
