@@ -136,6 +136,26 @@ typedef struct hvm_hip_out {
 /* One-shot: create a context, load, evaluate, read back, destroy. */
 int hvm_hip_run(const uint32_t* book_buf, size_t n_words, const hvm_hip_cfg* cfg, hvm_hip_out* out);
 
+/* No counterpart in the reference.  hvm_hip_run for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the net, bags, control blocks, book and
+ * readback lists live in a static device array of HVM_HIP_STATIC_BYTES that
+ * the library holds on each GPU until the process ends, so the call makes no
+ * hipMalloc/hipFree and creates no stream, event or graph (it runs on the
+ * default stream and times turns with the host clock).  Limits: node_cap and
+ * vars_cap at most HVM_HIP_STATIC_SLOTS (0 means that), bag_cap at most
+ * HVM_HIP_STATIC_BAG, books up to 4 MiB, results up to 2^18 nodes/vars; no
+ * HVM_HIP_WIDE, HVM_HIP_LOCAL_NET or HVM_HIP_GRAPHS.  Larger requests return
+ * HVM_HIP_E_ARG with the reason in hvm_hip_last_error(NULL).  The printed
+ * result goes to `result` (result_cap bytes, NUL-terminated; may be NULL with
+ * HVM_HIP_NO_READBACK); out->result and out->canonical are left NULL.  Calls
+ * on one GPU share its array: calls that overlap in time on one GPU race. */
+#define HVM_HIP_STATIC_BYTES (64u << 20)
+#define HVM_HIP_STATIC_SLOTS (1u << 21)
+#define HVM_HIP_STATIC_BAG   (1u << 20)
+int hvm_hip_run_static(const uint32_t* book_buf, size_t n_words, const hvm_hip_cfg* cfg, hvm_hip_out* out,
+                       char* result, size_t result_cap);
+
 /* Frees result/canonical strings and anything else this library returns. */
 void hvm_hip_free(void* p);
 

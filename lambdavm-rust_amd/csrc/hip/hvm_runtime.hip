@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cerrno>
+#include <mutex>
 #include <cstdio>
 #include <ctime>
 #include <cstdlib>
@@ -81,11 +82,52 @@ struct hvm_hip_ctx {
   double kernel_ms = 0, launch_ms = 0;
   u32 turns = 0, normalizes = 0, launches = 0;
   std::string last_error;
+  // No-allocation mode (hvm_hip_run_static): every device buffer is carved
+  // from a caller-provided region (the library's static scratch array); the
+  // default stream, no events, no graphs.
+  bool static_mode = false;
+  unsigned char* sbase = nullptr;
+  size_t scap = 0, soff = 0;
+  unsigned char* s_book = nullptr;     // book records + pairs
+  size_t s_book_bytes = 0;
+  unsigned char* s_gather = nullptr;   // readback lists
+  u32 s_gather_cap = 0;
+  void* carve(size_t bytes) {
+    size_t at = (soff + 255) & ~(size_t)255;
+    if (at + bytes > scap) return nullptr;
+    soff = at + bytes;
+    return sbase + at;
+  }
 
   bool fail(const std::string& m) { last_error = m; return false; }
   bool ck(hipError_t e, const char* what) {
     if (e != hipSuccess) return fail(std::string(what) + ": " + hipGetErrorString(e));
     return true;
+  }
+
+  // Limits of the no-allocation mode and the layout of its region: the net
+  // (HVM_HIP_STATIC_SLOTS nodes and vars), two bags of HVM_HIP_STATIC_BAG
+  // redexes, the book, the readback lists and the small control blocks.
+  static constexpr size_t kStaticBook = 4u << 20;
+  static constexpr u32 kStaticGather = 1u << 18;
+
+  bool init_static(const hvm_hip_cfg* c, void* base, size_t bytes) {
+    static_mode = true;
+    sbase = (unsigned char*)base;
+    scap = bytes;
+    soff = 0;
+    if (c) cfg = *c;
+    if (cfg.flags & (HVM_HIP_WIDE | HVM_HIP_LOCAL_NET | HVM_HIP_GRAPHS))
+      return fail("hvm_hip_run_static: HVM_HIP_WIDE, HVM_HIP_LOCAL_NET and HVM_HIP_GRAPHS need allocations");
+    if (cfg.node_cap > HVM_HIP_STATIC_SLOTS || cfg.vars_cap > HVM_HIP_STATIC_SLOTS)
+      return fail("hvm_hip_run_static: node_cap/vars_cap above HVM_HIP_STATIC_SLOTS (" +
+                  std::to_string(HVM_HIP_STATIC_SLOTS) + ")");
+    if (cfg.bag_cap > HVM_HIP_STATIC_BAG)
+      return fail("hvm_hip_run_static: bag_cap above HVM_HIP_STATIC_BAG (" + std::to_string(HVM_HIP_STATIC_BAG) + ")");
+    if (!cfg.node_cap) cfg.node_cap = HVM_HIP_STATIC_SLOTS;
+    if (!cfg.vars_cap) cfg.vars_cap = HVM_HIP_STATIC_SLOTS;
+    if (!cfg.bag_cap) cfg.bag_cap = HVM_HIP_STATIC_BAG;
+    return init(&cfg);
   }
 
   bool init(const hvm_hip_cfg* c) {
@@ -149,6 +191,30 @@ struct hvm_hip_ctx {
     node_pool_hi = node_cap - std::min<u32>(1u << 16, (node_cap - node_pool_lo) / 2);
     vars_pool_hi = vars_cap - std::min<u32>(1u << 16, (vars_cap - vars_pool_lo) / 2);
     if (node_slice < 4 * HVM_WAVE || vars_slice < 4 * HVM_WAVE) return fail("net too small for the wave count");
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) == hipSuccess && khz > 0) clock_khz = (uint32_t)khz;
+    memset(&host_sched, 0, sizeof host_sched);
+    host_sched.mode = HVM_SCHED_PLAIN;
+    if (static_mode) {
+      // every buffer from the region; the default stream; no events
+      node = (u64*)carve(((size_t)node_cap + 1) * 8);
+      vars = (u32*)carve(((size_t)vars_cap + 1) * 4);
+      bag[0] = (u64*)carve((size_t)bag_cap * 8);
+      bag[1] = (u64*)carve((size_t)bag_cap * 8);
+      ctl = (HvmCtl*)carve(sizeof(HvmCtl));
+      wave_state = (HvmWaveState*)carve(nwaves * sizeof(HvmWaveState));
+      stats = (HvmDevStats*)carve(sizeof(HvmDevStats));
+      sched = (HvmSched*)carve(sizeof(HvmSched));
+      d_params = (unsigned short*)carve(HVM_MLP_PARAMS * sizeof(unsigned short));
+      s_book = (unsigned char*)carve(kStaticBook);
+      s_book_bytes = kStaticBook;
+      s_gather_cap = kStaticGather;
+      s_gather = (unsigned char*)carve((size_t)kStaticGather * (4 + 8 + 4 + 4 + 4) + 16);
+      if (!node || !vars || !bag[0] || !bag[1] || !ctl || !wave_state || !stats || !sched || !d_params || !s_book ||
+          !s_gather)
+        return fail("hvm_hip_run_static: the static region is too small for this configuration");
+      return true;
+    }
     if (!ck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
     if (!ck(hipEventCreate(&ev0), "hipEventCreate") || !ck(hipEventCreate(&ev1), "hipEventCreate")) return false;
     // one spare slot past each capacity: the evaluator's accesses through an
@@ -163,14 +229,11 @@ struct hvm_hip_ctx {
     if (!ck(hipMalloc(&stats, sizeof(HvmDevStats)), "hipMalloc(stats)")) return false;
     if (!ck(hipMalloc(&sched, sizeof(HvmSched)), "hipMalloc(sched)")) return false;
     if (!ck(hipMalloc(&d_params, HVM_MLP_PARAMS * sizeof(unsigned short)), "hipMalloc(params)")) return false;
-    int khz = 0;
-    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) == hipSuccess && khz > 0) clock_khz = (uint32_t)khz;
-    memset(&host_sched, 0, sizeof host_sched);
-    host_sched.mode = HVM_SCHED_PLAIN;
     return true;
   }
 
   void release() {
+    if (static_mode) return;   // the region is not ours; no stream, events or graphs
     if (wide) { wide->destroy(); delete wide; wide = nullptr; }
     // teardown: a failing free or destroy has nothing left to undo
     for (void* p : {(void*)node, (void*)vars, (void*)bag[0], (void*)bag[1], (void*)ctl, (void*)wave_state,
@@ -250,10 +313,19 @@ struct hvm_hip_ctx {
       for (u32 i = 0; i < HVM_PRE_NODE && i < x.h.nlen; ++i) x.node[i] = pairs[x.h.node_off + i];
       for (u32 i = 0; i < HVM_PRE_RBAG && i < x.h.rlen; ++i) x.rbag[i] = pairs[x.h.rbag_off + i];
     }
-    if (d_defs) { (void)hipFree(d_defs); d_defs = nullptr; }
-    if (d_pairs) { (void)hipFree(d_pairs); d_pairs = nullptr; }
-    if (!ck(hipMalloc(&d_defs, defx.size() * sizeof(HvmDevDefX)), "hipMalloc(defs)")) return false;
-    if (!ck(hipMalloc(&d_pairs, pairs.size() * 8), "hipMalloc(pairs)")) return false;
+    if (static_mode) {
+      const size_t dbytes = (defx.size() * sizeof(HvmDevDefX) + 255) & ~(size_t)255;
+      if (dbytes + pairs.size() * 8 > s_book_bytes)
+        return fail("hvm_hip_run_static: the book exceeds the static region's " + std::to_string(s_book_bytes) +
+                    " bytes for books");
+      d_defs = (HvmDevDefX*)s_book;
+      d_pairs = (u64*)(s_book + dbytes);
+    } else {
+      if (d_defs) { (void)hipFree(d_defs); d_defs = nullptr; }
+      if (d_pairs) { (void)hipFree(d_pairs); d_pairs = nullptr; }
+      if (!ck(hipMalloc(&d_defs, defx.size() * sizeof(HvmDevDefX)), "hipMalloc(defs)")) return false;
+      if (!ck(hipMalloc(&d_pairs, pairs.size() * 8), "hipMalloc(pairs)")) return false;
+    }
     if (!ck(hipMemcpy(d_defs, defx.data(), defx.size() * sizeof(HvmDevDefX), hipMemcpyHostToDevice), "copy defs")) return false;
     if (!ck(hipMemcpy(d_pairs, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice), "copy pairs")) return false;
     ndefs = (u32)defs.size();
@@ -328,6 +400,7 @@ struct hvm_hip_ctx {
     if (mode == HVM_SCHED_LEARNED && !params) return fail("learned scheduler needs parameters");
     if (params && !ck(hipMemcpy(d_params, params, HVM_MLP_PARAMS * sizeof(unsigned short), hipMemcpyHostToDevice),
                       "params upload")) return false;
+    if (static_mode && trace_cap) return fail("hvm_hip_run_static: no scheduler trace without allocations");
     if (trace_cap != host_sched.trace_cap) {
       if (d_trace) { (void)hipFree(d_trace); d_trace = nullptr; }
       if (trace_cap && !ck(hipMalloc(&d_trace, (size_t)trace_cap * HVM_TRACE_REC * sizeof(float)), "hipMalloc(trace)"))
@@ -394,11 +467,12 @@ struct hvm_hip_ctx {
   // replayed, or launched directly.
   bool launch_chunk(const HvmTurnArgs& a, u32 chunk) {
     const bool graphs = (cfg.flags & HVM_HIP_GRAPHS) != 0;
+    const bool events = !graphs && !static_mode;
     auto direct = [&](hipStream_t st) {
       for (u32 i = 0; i < chunk; ++i) {
-        if (!graphs && !ck(hipEventRecord(tev[2 * i], st), "record turn start")) return false;
+        if (events && !ck(hipEventRecord(tev[2 * i], st), "record turn start")) return false;
         if (!ck(hvm_launch_turn(&a, st), "launch turn")) return false;
-        if (!graphs && !ck(hipEventRecord(tev[2 * i + 1], st), "record turn end")) return false;
+        if (events && !ck(hipEventRecord(tev[2 * i + 1], st), "record turn end")) return false;
         if (!ck(hvm_launch_between(&a, clock_khz, st), "launch between")) return false;
       }
       return true;
@@ -466,7 +540,7 @@ struct hvm_hip_ctx {
     HvmTurnArgs a = turn_args();
     u32 chunk = 2;
     const u32 max_chunk = 64;
-    if (tev.size() < 2 * max_chunk) {
+    if (!static_mode && tev.size() < 2 * max_chunk) {
       for (size_t i = tev.size(); i < 2 * max_chunk; ++i) {
         hipEvent_t e;
         if (!ck(hipEventCreate(&e), "hipEventCreate")) return false;
@@ -476,11 +550,17 @@ struct hvm_hip_ctx {
     u32 turns_before = host_ctl.turns;
     for (;;) {
       if (deadline > 0 && now_s() > deadline) return fail("time limit exceeded");
+      const double t_chunk = now_s();
       if (!launch_chunk(a, chunk)) return false;
       if (!ck(hipMemcpyAsync(&host_ctl, ctl, sizeof(HvmCtl), hipMemcpyDeviceToHost, stream), "ctl readback")) return false;
       if (!ck(hipStreamSynchronize(stream), "turn sync")) return false;
       u32 ran = host_ctl.turns - turns_before;
-      if (cfg.flags & HVM_HIP_GRAPHS) {
+      if (static_mode) {
+        // no events: the chunk's host time, shared by the turns that ran
+        const double ms = (now_s() - t_chunk) * 1e3;
+        kernel_ms += ms * (double)ran / (double)chunk;
+        launch_ms += ms;
+      } else if (cfg.flags & HVM_HIP_GRAPHS) {
         // one span for the chunk (its between kernels included)
         float ms = 0;
         if (!ck(hipEventElapsedTime(&ms, tev[0], tev[1]), "chunk time")) return false;
@@ -816,16 +896,27 @@ struct hvm_hip_ctx {
   // Reads back everything reachable from `port` and prints it the way
   // pretty_print_port does (src/hvm.c:1618-1699).
   bool print_result(u32 port, std::string& out) {
-    u32 cap = 1u << 20;
-    for (int attempt = 0; attempt < 6; ++attempt, cap <<= 2) {
+    u32 cap = static_mode ? s_gather_cap : 1u << 20;
+    for (int attempt = 0; attempt < (static_mode ? 1 : 6); ++attempt, cap <<= 2) {
       u32 *d_nidx = nullptr, *d_vidx = nullptr, *d_vval = nullptr, *d_q = nullptr, *d_cnt = nullptr;
       u64* d_nval = nullptr;
-      bool ok = ck(hipMalloc(&d_nidx, (size_t)cap * 4), "gather alloc") &&
-                ck(hipMalloc(&d_nval, (size_t)cap * 8), "gather alloc") &&
-                ck(hipMalloc(&d_vidx, (size_t)cap * 4), "gather alloc") &&
-                ck(hipMalloc(&d_vval, (size_t)cap * 4), "gather alloc") &&
-                ck(hipMalloc(&d_q, (size_t)cap * 4), "gather alloc") &&
-                ck(hipMalloc(&d_cnt, 16), "gather alloc");
+      bool ok = true;
+      if (static_mode) {
+        // the readback lists live in the region: node values first (8-byte aligned)
+        d_nval = (u64*)s_gather;
+        d_nidx = (u32*)(s_gather + (size_t)cap * 8);
+        d_vidx = d_nidx + cap;
+        d_vval = d_vidx + cap;
+        d_q = d_vval + cap;
+        d_cnt = d_q + cap;
+      } else {
+        ok = ck(hipMalloc(&d_nidx, (size_t)cap * 4), "gather alloc") &&
+             ck(hipMalloc(&d_nval, (size_t)cap * 8), "gather alloc") &&
+             ck(hipMalloc(&d_vidx, (size_t)cap * 4), "gather alloc") &&
+             ck(hipMalloc(&d_vval, (size_t)cap * 4), "gather alloc") &&
+             ck(hipMalloc(&d_q, (size_t)cap * 4), "gather alloc") &&
+             ck(hipMalloc(&d_cnt, 16), "gather alloc");
+      }
       u32 cnt[3] = {0, 0, 0};
       std::vector<u32> nidx, vidx_, vval;
       std::vector<u64> nval;
@@ -849,8 +940,10 @@ struct hvm_hip_ctx {
         }
       }
       // scratch of this readback: nothing to undo if a free fails
-      (void)hipFree(d_nidx); (void)hipFree(d_nval); (void)hipFree(d_vidx); (void)hipFree(d_vval);
-      (void)hipFree(d_q); (void)hipFree(d_cnt);
+      if (!static_mode) {
+        (void)hipFree(d_nidx); (void)hipFree(d_nval); (void)hipFree(d_vidx); (void)hipFree(d_vval);
+        (void)hipFree(d_q); (void)hipFree(d_cnt);
+      }
       if (!ok) return false;
       if (cnt[2] & 1) return fail("readback reached a port outside the net");
       if (cnt[2] & 2 || cnt[0] > cap || cnt[1] > cap) continue;  // grow and retry
@@ -900,7 +993,9 @@ struct hvm_hip_ctx {
       }
       return true;
     }
-    return fail("result too large to read back");
+    return fail(static_mode ? "result too large for the static readback region (" + std::to_string(s_gather_cap) +
+                                  " nodes/vars)"
+                            : "result too large to read back");
   }
 
   // The 64-bit-port evaluator: boot, normalise, read back (no IO driver).
@@ -1182,6 +1277,80 @@ extern "C" int hvm_hip_run(const uint32_t* book_buf, size_t n_words, const hvm_h
   int rc = c->load(book_buf, n_words) ? HVM_HIP_OK : HVM_HIP_E_BOOK;
   if (rc == HVM_HIP_OK) rc = c->eval(out);
   hvm_hip_destroy(c);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation entry (hvm_hip_run_static): the net, bags, control blocks,
+// book and readback lists live in `scratch`, a static device array of
+// HVM_HIP_STATIC_BYTES that each GPU the library runs on holds until the
+// process ends.  Its device address is looked up once per device with that
+// device current; host code passes that address, never the array's name.
+// Every call on a device shares that device's array: calls that overlap on
+// one device race, and each call clears what must start at zero.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) unsigned char scratch[HVM_HIP_STATIC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// The device address of this device's scratch array (device `dev` current).
+bool scratch_ptr(int dev, void** out, std::string& err) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) { err = "device index out of range"; return false; }
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return false;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return true;
+}
+}  // namespace
+
+extern "C" int hvm_hip_run_static(const uint32_t* book_buf, size_t n_words, const hvm_hip_cfg* cfg, hvm_hip_out* out,
+                                  char* result, size_t result_cap) {
+  if (!book_buf || !out) { g_create_error = "hvm_hip_run_static: NULL book or output"; return HVM_HIP_E_ARG; }
+  memset(out, 0, sizeof *out);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return HVM_HIP_E_DEVICE; }
+  if (hipSetDevice(0) != hipSuccess) { g_create_error = "hipSetDevice failed"; return HVM_HIP_E_DEVICE; }
+  void* base = nullptr;
+  if (!scratch_ptr(0, &base, g_create_error)) return HVM_HIP_E_DEVICE;
+  hvm_hip_cfg c{};
+  if (cfg) c = *cfg;
+  hvm_hip_ctx ctx;
+  int rc = HVM_HIP_OK;
+  if (!ctx.init_static(&c, base, HVM_HIP_STATIC_BYTES)) {
+    rc = ctx.last_error.rfind("hvm_hip_run_static", 0) == 0 ? HVM_HIP_E_ARG : HVM_HIP_E_DEVICE;
+  } else if (!ctx.load(book_buf, n_words)) {
+    rc = ctx.last_error.find("static region") != std::string::npos ? HVM_HIP_E_ARG : HVM_HIP_E_BOOK;
+  } else {
+    rc = ctx.eval(out);
+  }
+  if (rc == HVM_HIP_OK) {
+    // the printed result goes to the caller's buffer
+    const char* text = out->result ? out->result : "";
+    const size_t n = strlen(text);
+    if (result && result_cap > n) {
+      memcpy(result, text, n + 1);
+    } else if (result || !(c.flags & HVM_HIP_NO_READBACK)) {
+      ctx.last_error = "hvm_hip_run_static: result buffer too small (" + std::to_string(n + 1) + " bytes needed)";
+      rc = HVM_HIP_E_ARG;
+    }
+  }
+  free(out->result);
+  free(out->canonical);
+  out->result = out->canonical = nullptr;
+  if (rc != HVM_HIP_OK) g_create_error = ctx.last_error;
+  ctx.release();
   return rc;
 }
 
