@@ -126,7 +126,8 @@ struct HvmCtl {
   uint32_t fixed_steps; // if nonzero, every turn runs this many steps
   uint32_t idle_waves;  // waves of the current turn that ran out of work
   uint32_t idle_quit;   // the current turn ends once this many waves are idle (0: never early)
-  uint32_t pad[5];
+  uint32_t surplus;     // some wave of the current turn held more redexes than two per lane
+  uint32_t pad[4];
 };
 
 // ---------------------------------------------------------------------------

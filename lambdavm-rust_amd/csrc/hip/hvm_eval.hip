@@ -673,8 +673,22 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     }
     if ((step & 15) == 15) {
       if (!evac) {
+        // A turn that started without idle quit (few redexes, not growing:
+        // independent sequential chains) gets it back once some wave holds
+        // more redexes than two per lane: work grew inside the turn, and
+        // only the end of the turn hands it to the idle waves.
+        u32 iq = idle_quit;
+        if (!iq && A.idle_quit) {
+          u32 mine = wave_sum64(ln.top - ln.bot);
+          u32 sur = __hip_atomic_load(&A.ctl->surplus, __ATOMIC_RELAXED, AGENT);
+          if (!sur && mine > 2u * HVM_WAVE) {
+            if (lane == 0) __hip_atomic_store(&A.ctl->surplus, 1u, __ATOMIC_RELAXED, AGENT);
+            sur = 1;
+          }
+          if (sur) iq = A.idle_quit;
+        }
         u32 idle = __hip_atomic_load(&A.ctl->idle_waves, __ATOMIC_RELAXED, AGENT);
-        if (idle_quit && idle >= idle_quit) {
+        if (iq && idle >= iq) {
           if (!LOC) break;
           evac = true;
         }
@@ -1461,6 +1475,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
         c->n_in = n;
         c->n_out = 0;
         c->idle_waves = 0;
+        c->surplus = 0;
         c->in_head = min(n, A.nwaves * (u32)HVM_WAVE);
         c->parity ^= 1u;
         s_n = n;

@@ -150,6 +150,7 @@ struct Model {
   std::vector<u64> bag[2];
   // control block
   u32 n_in = 0, n_out = 0, in_head = 0, err = 0, parity = 0, max_steps = 0, idle_waves = 0, idle_quit = 0;
+  u32 idle_cfg = 0, surplus = 0;
   u32 node_slice = 0, vars_slice = 0, node_pool_lo = 0, vars_pool_lo = 0, node_pool_hi = 0, vars_pool_hi = 0;
   std::vector<WaveState> ws;
   Rng rng;
@@ -497,7 +498,16 @@ void Model::turn(u32 w) {
     }
     if ((step & 15) == 15) {
       yield();
-      if (idle_quit && idle_waves >= idle_quit) break;
+      // a steady turn (no idle quit) gets the configured idle quit back once
+      // some wave holds more redexes than two per lane (the kernel's rule)
+      u32 iq = idle_quit;
+      if (!iq && idle_cfg) {
+        u32 mine = 0;
+        for (auto& ln : lanes) mine += ln.top - ln.bot;
+        if (mine > 2 * L) surplus = 1;
+        if (surplus) iq = idle_cfg;
+      }
+      if (iq && idle_waves >= iq) break;
       if ((step & 63) == 63 && err) break;
     }
     // ---- P0: peek + classify ----
@@ -901,9 +911,11 @@ extern "C" int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_
   const u32 idle_cfg = pct ? std::max<u32>(1u, (C.waves * pct) / 100) : 0u;
   m.max_steps = C.turn_steps ? C.turn_steps : 4096;
   m.idle_quit = idle_cfg;
+  m.idle_cfg = idle_cfg;
   for (u32 t = 0; t < (1u << 20); ++t) {
     m.n_out = 0;
     m.idle_waves = 0;
+    m.surplus = 0;
     m.run_turn();
     out->turns++;
     if (m.abandoned || m.err) break;
