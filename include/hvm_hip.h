@@ -193,26 +193,36 @@ int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_net
 /* ------------------------------------------------------------------------ */
 
 /* After every turn the scheduler picks how the next turn shares work: one of
- * 4 actions (turn length in steps, idle-quit share of the waves):
- * (4096, 10%), (4096, idle_pct), (4096, 50%), (512, idle_pct) — how long
- * waves work on their own redexes, and how many of them must run dry, before
- * the HBM bag redistributes redexes.  It never changes what an interaction
- * does, so normal form and ITRS do not depend on it.
- *   HVM_HIP_SCHED_PLAIN   action 1: the longest turn, cut short once
- *                         hvm_hip_cfg.idle_pct of the waves are idle
+ * 4 actions (wave share, idle-quit share) = (all, idle_pct), (all, 50%),
+ * (1/2, idle_pct), (all, 10%).  A turn runs on that share of the waves
+ * (the others end at once; fewer, fuller waves for a small frontier) and ends
+ * once its idle-quit share of them has run out of work, when the HBM bag
+ * redistributes the redexes; turns are at most 4096 steps.  idle_pct is
+ * hvm_hip_cfg.idle_pct (25% by default).  A turn on part of the waves
+ * allocates from those waves' slices and the shared pool only.  The choice
+ * never changes what an interaction does, so normal form and ITRS do not
+ * depend on it.
+ *   HVM_HIP_SCHED_PLAIN   action 0, every turn
  *   HVM_HIP_SCHED_LEARNED argmax of a 32->64->64->4 MLP (fp16 params,
- *                         HVM_HIP_MLP_PARAMS values) evaluated on the device;
- *                         it replaces the heuristic's choice only when its
- *                         predicted reward is higher by more than
- *                         explore_pct/100 (the heuristic stays the fallback)
- *   HVM_HIP_SCHED_EXPLORE the heuristic with explore_pct% random choices
+ *                         HVM_HIP_MLP_PARAMS values) evaluated on the device
+ *                         from the finished turn's features (bag sizes, lane
+ *                         use, interaction mix, how the turn ended, per-wave
+ *                         occupancy at its end, free-slot density, rate);
+ *                         it replaces action 0 only when its predicted
+ *                         value is higher by more than explore_pct/100
+ *                         (the plain choice stays the fallback)
+ *   HVM_HIP_SCHED_EXPLORE action 0 with explore_pct% random choices
  *                         (seeded), for collecting training traces
+ *   HVM_HIP_SCHED_FIXED   action explore_pct % 4, every turn
+ * A nonzero seed rotates the initial placement of redexes over the waves and
+ * shortens each turn by up to an eighth, in every mode.
  * trace_cap > 0 records one HVM_HIP_TRACE_REC-float record per decision:
- * 32 features, action, reward (log2 interactions per µs of the turn the
- * action governed), interactions, µs. */
+ * 32 features, action, log2 interactions per µs of the turn the action
+ * governed, its interactions, its µs. */
 #define HVM_HIP_SCHED_PLAIN   0u
 #define HVM_HIP_SCHED_LEARNED 1u
 #define HVM_HIP_SCHED_EXPLORE 2u
+#define HVM_HIP_SCHED_FIXED   3u
 #define HVM_HIP_MLP_FEAT   32
 #define HVM_HIP_MLP_PARAMS (32 * 64 + 64 + 64 * 64 + 64 + 64 * 4 + 4)
 #define HVM_HIP_TRACE_REC  36

@@ -1,6 +1,6 @@
 """Autotuner for the evaluator's launch and scheduling knobs.
 
-    python lambdavm-rust_amd/autotune.py [--workload batch|bitonic|radix|sum|church]
+    python lambdavm-rust_amd/autotune.py [--workload batch|radix_lsd|church_loop|bitonic|radix|sum|church]
         [--repeats 2] [--out lambdavm-rust_amd/tuned_<workload>.json]
 
 The knobs are the ones a caller can set per context (include/hvm_hip.h,
@@ -15,9 +15,17 @@ hvm_hip_cfg):
   * flags        — HVM_HIP_LOCAL_NET (per-wave LDS nets) and/or
                    HVM_HIP_COMPILED_CALLS (def templates pre-reduced at load);
   * graphs       — HVM_HIP_GRAPHS (turn chunks replayed from HIP graphs) or
-                   not, or-ed into the flags.
-The per-lane redex deque (HVM_DEQ) and the LDS net sizes are compile-time
-constants of the kernels and are not searched here.
+                   not, or-ed into the flags;
+and the turn kernel's compile-time knobs, through variant builds of the
+library (lib/variants/libhvm_hip_<name>.so, `make -C lambdavm-rust_amd
+variants`), each measured in a child process that loads it (HVM_HIP_LIB):
+  * variant      — deq16 / deq64: per-lane redex deque of 16 / 64 entries
+                   instead of 32 (LDS per wave, hence waves per CU);
+                   scan4 / scan32: slice windows an allocation scans before
+                   the shared pool (4 / 32 instead of 8); lnet512 / lnet2048:
+                   local-net sizes of the local-net evaluator (1024 default),
+                   tried only when that evaluator is selected.
+The kernel runs one wave per workgroup; waves per workgroup is not a knob.
 
 Search: coordinate descent from the defaults, one knob at a time, keeping a
 value only when its median time beats the best so far.  Every candidate must
@@ -32,9 +40,11 @@ and tuned times, and every measurement.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -46,10 +56,11 @@ import programs as P  # noqa: E402
 
 
 class Workload:
-    """A book plus how to evaluate it once on a context."""
+    """A book plus how to evaluate it once on a context; `key` names it for
+    workload() (child processes rebuild it from the key)."""
 
-    def __init__(self, name, words, runner, itrs=None):
-        self.name, self.words, self.runner, self.itrs = name, words, runner, itrs
+    def __init__(self, name, words, runner, itrs=None, key=None):
+        self.name, self.words, self.runner, self.itrs, self.key = name, words, runner, itrs, key
 
 
 def single(name: str, code: str) -> Workload:
@@ -63,14 +74,54 @@ def batched(seed: int = 0, n: int = 256) -> Workload:
     entries = [fids[f"net{i}"] for i in range(len(nets))]
     est = [P.batch_estimate(gen, params) for gen, params, _, _ in nets]
     words = H.compile_book(code)
-    return Workload(f"batch(seed={seed}, n={n})", words, lambda ctx: ctx.eval_batch(entries, est)[1])
+    w = Workload(f"batch(seed={seed}, n={n})", words, None)
+
+    def run(ctx):
+        per, tot = ctx.eval_batch(entries, est)
+        w.last_per = per
+        return tot
+    w.runner = run
+    return w
+
+
+VARIANTS = ("deq16", "deq64", "scan4", "scan32", "lnet512", "lnet2048")
+
+
+def variant_lib(name: str) -> str:
+    return os.path.join(HERE, "lib", "variants", f"libhvm_hip_{name}.so")
 
 
 def measure(w: Workload, cfg: dict, repeats: int, time_limit_s: int = 60):
     """Median eval time (ms) of `repeats` runs after one warm-up; also the
-    identity (ITRS, canonical NF) of the result."""
-    flags = cfg.get("flags", 0) | cfg.get("graphs", 0) | (H.NO_READBACK if w.name.startswith("batch") else 0)
-    ctx_cfg = {k: v for k, v in cfg.items() if k != "graphs"}
+    identity (ITRS, canonical NF) of the result.  A variant build is measured
+    in a child process that loads that library."""
+    if cfg.get("variant"):
+        return measure_child(w, cfg, repeats, time_limit_s)
+    return measure_here(w, cfg, repeats, time_limit_s)
+
+
+def measure_child(w: Workload, cfg: dict, repeats: int, time_limit_s: int = 60):
+    lib = variant_lib(cfg["variant"])
+    if not os.path.exists(lib):
+        raise H.HvmError(f"{lib} is missing: build it with `make -C lambdavm-rust_amd variants`")
+    env = dict(os.environ, HVM_HIP_LIB=lib)
+    p = subprocess.run([sys.executable, os.path.abspath(__file__), "--child", "--workload", w.key,
+                        "--cfg", json.dumps(cfg), "--repeats", str(repeats), "--time-limit", str(time_limit_s)],
+                       capture_output=True, text=True, env=env, timeout=time_limit_s * (repeats + 2) + 120)
+    line = (p.stdout.strip().splitlines() or [""])[-1]
+    try:
+        res = json.loads(line)
+    except ValueError:
+        raise RuntimeError(f"variant child failed under {cfg}: {p.stderr[-800:]}")
+    if "error" in res:
+        raise (H.HvmEvalError(res["error"], res.get("err", 0)) if res.get("eval") else H.HvmError(res["error"]))
+    return res["ms"], tuple(res["ident"]), None
+
+
+def measure_here(w: Workload, cfg: dict, repeats: int, time_limit_s: int = 60, readback: bool = False):
+    flags = cfg.get("flags", 0) | cfg.get("graphs", 0) | \
+        (H.NO_READBACK if w.name.startswith("batch") and not readback else 0)
+    ctx_cfg = {k: v for k, v in cfg.items() if k not in ("graphs", "variant")}
     with H.Context(time_limit_s=time_limit_s, **dict(ctx_cfg, flags=flags)) as ctx:
         ctx.load(w.words)
         first = w.runner(ctx)
@@ -81,15 +132,16 @@ def measure(w: Workload, cfg: dict, repeats: int, time_limit_s: int = 60):
             ms.append((time.perf_counter() - t0) * 1e3)
             if r.itrs != first.itrs:
                 raise RuntimeError(f"ITRS changed between runs under {cfg}: {first.itrs} vs {r.itrs}")
-    ident = (first.itrs, None if w.name.startswith("batch") else first.canonical)
-    return statistics.median(ms), ident, first
+    ident = (first.itrs, None if w.name.startswith("batch") else
+             hashlib.sha256(first.canonical.encode()).hexdigest())
+    return (statistics.median(ms) if ms else 0.0), ident, first
 
 
 def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
     """Coordinate descent over the knobs; returns the report dict."""
     base = dict(base_cfg or {})
     t_def, ident, r0 = measure(w, base, repeats)
-    waves_def = r0.diag.get("waves", 0)
+    waves_def = r0.diag.get("waves", 0) if r0 is not None else 0
     log(f"[{w.name}] default: {t_def:.2f} ms, waves {waves_def}, itrs {ident[0]}")
     space = space or [
         # fractions of the default (all resident waves)
@@ -98,6 +150,7 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
         ("turn_steps", [96, 512, 4096, 0]),
         ("flags", [H.COMPILED_CALLS, H.LOCAL_NET, H.LOCAL_NET | H.COMPILED_CALLS, 0]),
         ("graphs", [H.GRAPHS, 0]),
+        ("variant", ["deq16", "deq64", "scan4", "scan32", "lnet512", "lnet2048", ""]),
     ]
     best, t_best = dict(base), t_def
     rows = [{"cfg": dict(base), "ms": round(t_def, 3)}]
@@ -107,6 +160,10 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
             cand[knob] = v
             if knob == "flags" and v & H.LOCAL_NET:
                 cand["waves"] = 0   # its own resident-wave count
+            if knob == "variant" and v.startswith("lnet") and not cand.get("flags", 0) & H.LOCAL_NET:
+                continue            # local-net sizes matter to the local-net evaluator only
+            if not cand.get("variant"):
+                cand.pop("variant", None)
             if cand == best:
                 continue
             try:
@@ -139,9 +196,19 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
 
 def workload(name: str):
     """The workload and the configuration the search starts from."""
+    w, base = _workload(name)
+    w.key = name
+    return w, base
+
+
+def _workload(name: str):
     if name == "batch":
         # 256 nets of up to 2^22 leaves need more than the default 2^28 slots
         return batched(), {"node_cap": 1 << 29, "vars_cap": 1 << 29}
+    if name == "radix_lsd":
+        return single("radix_lsd(18, 0)", P.radix_lsd(18, 0)[0]), {}
+    if name == "church_loop":
+        return single("church_loop(12, 12)", P.church_loop(12, 12)[0]), {}
     if name == "bitonic":
         return single("bitonic(16, 0)", P.bitonic(16, 0)[0]), {}
     if name == "radix":
@@ -153,12 +220,36 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def child(args):
+    """One measurement in this process (the library named by HVM_HIP_LIB);
+    prints one JSON line."""
+    w, _ = workload(args.workload)
+    cfg = json.loads(args.cfg)
+    try:
+        ms, ident, _ = measure_here(w, cfg, args.repeats, args.time_limit, readback=args.readback)
+        out = {"ms": ms, "ident": list(ident), "lib": H.LIB_PATH}
+        per = getattr(w, "last_per", None)
+        if per is not None and args.readback:
+            out["nets_nf_sha256"] = [hashlib.sha256((p.canonical or "").encode()).hexdigest() for p in per]
+        print(json.dumps(out), flush=True)
+    except H.HvmEvalError as e:
+        print(json.dumps({"error": str(e), "err": e.err, "eval": True}), flush=True)
+    except H.HvmError as e:
+        print(json.dumps({"error": str(e)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="batch")
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cfg", default="{}", help=argparse.SUPPRESS)
+    ap.add_argument("--time-limit", type=int, default=60, help=argparse.SUPPRESS)
+    ap.add_argument("--readback", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.child:
+        return child(args)
     w, base = workload(args.workload)
     rep = tune(w, repeats=args.repeats, log=lambda s: print(s, flush=True), base_cfg=base)
     out = args.out or os.path.join(HERE, f"tuned_{args.workload}.json")

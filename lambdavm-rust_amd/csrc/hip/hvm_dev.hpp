@@ -127,7 +127,9 @@ struct HvmCtl {
   uint32_t idle_waves;  // waves of the current turn that ran out of work
   uint32_t idle_quit;   // the current turn ends once this many waves are idle (0: never early)
   uint32_t surplus;     // some wave of the current turn held more redexes than two per lane
-  uint32_t pad[4];
+  uint32_t active;      // waves that run the current turn (0: all); the others end at once
+  uint32_t idle_back;   // idle quit a turn started without gets back on surplus (0: never)
+  uint32_t pad[2];
 };
 
 // ---------------------------------------------------------------------------
@@ -136,7 +138,7 @@ struct HvmCtl {
 // an interaction does, only when redexes are shared.
 // ---------------------------------------------------------------------------
 
-#define HVM_ACTIONS     4      // turn lengths the scheduler chooses from
+#define HVM_ACTIONS     4      // (wave share, idle-quit share) pairs the scheduler chooses from
 #define HVM_FEAT        32     // MLP input features
 #define HVM_HID         64     // MLP hidden width
 // packed fp16 parameters: W1[32][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4]
@@ -144,7 +146,7 @@ struct HvmCtl {
 // trace record: features, action, reward (log2 interactions per µs), itrs, dt µs
 #define HVM_TRACE_REC   (HVM_FEAT + 4)
 
-enum { HVM_SCHED_PLAIN = 0, HVM_SCHED_LEARNED = 1, HVM_SCHED_EXPLORE = 2 };
+enum { HVM_SCHED_PLAIN = 0, HVM_SCHED_LEARNED = 1, HVM_SCHED_EXPLORE = 2, HVM_SCHED_FIXED = 3 };
 
 // Per-turn counters, summed by the turn kernel and consumed (then cleared)
 // by the between kernel.
@@ -156,12 +158,17 @@ struct HvmTurnStats {
   unsigned long long defers;
   unsigned long long probes;
   unsigned long long kind[HVM_K_KINDS];
+  // per-wave occupancy at the end of the turn: waves that left 0, 1-64,
+  // 65-128 and more than 128 redexes (two per lane) to the output bag
+  unsigned long long wave_left[4];
+  unsigned long long alloc_slots;  // slots the allocator handed out (over probes * 64: free density)
 };
 
 struct HvmSched {
   uint32_t mode;                 // HVM_SCHED_*
   uint32_t seed;                 // exploration jitter
-  uint32_t explore_pct;          // % of turns with a random action (explore mode)
+  uint32_t explore_pct;          // % of turns with a random action (explore mode), the margin
+                                 // (learned mode) or the action (fixed mode)
   uint32_t trace_cap;            // records the trace buffer holds
   uint32_t trace_len;            // records written
   uint32_t last_action;

@@ -537,9 +537,14 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   // the first turn and by hvm_between_kernel after every turn).
   const HvmCtl* C = A.ctl;
   if (C->done) return;
+  // A turn runs on the first `act` waves (the scheduler's choice); the others
+  // end at once, their slices untouched.
+  const u32 act = C->active ? min(C->active, A.nwaves) : A.nwaves;
+  if (w >= act) return;
   const u32 n_in = C->n_in;
   const u32 max_steps = C->max_steps;
   const u32 idle_quit = C->idle_quit;
+  const u32 idle_back = C->idle_back;
   const u64* bag_in = A.bag[C->parity];
   u64* bag_out = A.bag[C->parity ^ 1u];
 
@@ -554,7 +559,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   ln.spec = 0;
   ln.fsw0 = ln.fswn = 0;
   ln.voids = 0;
-  u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0;
+  u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0, alloc_slots = 0;
+  u32 left_tot = 0;   // redexes this wave leaves to the output bag
 
   if (LOC) {
     // an empty local net: every slot free
@@ -582,11 +588,11 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   // not depend on the placement.
   {
     const u32 seed = A.sched->seed;
-    const u32 rot = seed ? sched_hash(seed, C->turns) % A.nwaves : 0u;
-    u32 idx = (w + rot) % A.nwaves + lane * A.nwaves;
-    if (idx < n_in && idx < HVM_WAVE * A.nwaves) push_redex<LOC>(L, N, ln, A, bag_in[idx]);
+    const u32 rot = seed ? sched_hash(seed, C->turns) % act : 0u;
+    u32 idx = (w + rot) % act + lane * act;
+    if (idx < n_in && idx < HVM_WAVE * act) push_redex<LOC>(L, N, ln, A, bag_in[idx]);
   }
-  bool more = n_in > HVM_WAVE * A.nwaves;
+  bool more = n_in > HVM_WAVE * act;
   // With the local net a wave ends its turn by draining the net (evacuation):
   // high-priority redexes run, every other redex leaves through the output bag.
   bool evac = false;
@@ -678,14 +684,14 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         // more redexes than two per lane: work grew inside the turn, and
         // only the end of the turn hands it to the idle waves.
         u32 iq = idle_quit;
-        if (!iq && A.idle_quit) {
+        if (!iq && idle_back) {
           u32 mine = wave_sum64(ln.top - ln.bot);
           u32 sur = __hip_atomic_load(&A.ctl->surplus, __ATOMIC_RELAXED, AGENT);
           if (!sur && mine > 2u * HVM_WAVE) {
             if (lane == 0) __hip_atomic_store(&A.ctl->surplus, 1u, __ATOMIC_RELAXED, AGENT);
             sur = 1;
           }
-          if (sur) iq = A.idle_quit;
+          if (sur) iq = idle_back;
         }
         u32 idle = __hip_atomic_load(&A.ctl->idle_waves, __ATOMIC_RELAXED, AGENT);
         if (iq && idle >= iq) {
@@ -831,6 +837,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     }
     u32 need_n = wave_max(run ? off_n + nn : 0);
     u32 need_v = wave_max(run ? off_v + nv : 0);
+    alloc_slots += need_n + need_v;
     u32 need_xn = 0, need_xv = 0;
     if (LOC && tot_x) {
       need_xn = wave_max(run ? off_xn + xn : 0);
@@ -1229,6 +1236,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     u32 n = ln.top - ln.bot;
     u32 tot;
     u32 off = wave_scan(n, &tot);
+    left_tot = tot;
     u32 base = 0;
     if (lane == 0 && tot) base = atomicAdd(&A.ctl->n_out, tot);
     base = __shfl(base, 0, HVM_WAVE);
@@ -1314,6 +1322,9 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       if (probes) atomicAdd(&T->probes, probes);
 #pragma unroll
       for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&T->kind[k], kinds[k]);
+      const u32 cls = left_tot == 0 ? 0u : left_tot <= HVM_WAVE ? 1u : left_tot <= 2u * HVM_WAVE ? 2u : 3u;
+      atomicAdd(&T->wave_left[cls], 1ull);
+      if (alloc_slots) atomicAdd(&T->alloc_slots, alloc_slots);
     }
   }
 }
@@ -1393,22 +1404,45 @@ __global__ __launch_bounds__(256) void hvm_gather_kernel(HvmGatherArgs G) {
 // all 64 lanes evaluate the MLP in learned mode.
 // ---------------------------------------------------------------------------
 
-// Scheduler actions: turn length and idle-quit share (0: the configured one,
-// hvm_hip_cfg.idle_pct).  Action 1 is the plain heuristic's.
-__device__ const u32 kTurnLen[HVM_ACTIONS] = {4096, 4096, 4096, 512};
-__device__ const u32 kIdlePct[HVM_ACTIONS] = {10, 0, 50, 0};
+// Scheduler actions: the share of the waves a turn runs on (1/div) and its
+// idle-quit share (0: the configured one, hvm_hip_cfg.idle_pct); turns are at
+// most 4096 steps.  Action 0 is the plain heuristic's.  Fewer waves put a
+// small frontier's redexes on fewer, fuller waves; a larger idle-quit share
+// lets waves run longer before the HBM bag redistributes.  (Shorter fixed
+// turns, 512 and 128 steps, were slower than idle quitting on every workload
+// measured, by up to 19%.)
+#ifndef HVM_SCHED_WDIV
+#define HVM_SCHED_WDIV 1, 1, 2, 1
+#endif
+#ifndef HVM_SCHED_IDLE
+#define HVM_SCHED_IDLE 0, 50, 0, 10
+#endif
+#define HVM_TURN_LEN 4096u
+__device__ const u32 kWaveDiv[HVM_ACTIONS] = {HVM_SCHED_WDIV};
+__device__ const u32 kIdlePct[HVM_ACTIONS] = {HVM_SCHED_IDLE};
 
-// The fixed heuristic: short turns while spreading a few redexes over the
-// waves, medium while growing, long once every lane can have work.
-// The plain heuristic: the longest turn with the configured idle-quit share.
-// A turn ends once that share of the waves has run out of work, which
-// shortens turns by itself when redexes are scarce (measured: shorter fixed
-// turns were slower on every workload here, by up to 19%).
+// The plain heuristic: every wave, the longest turn, ended early once the
+// configured idle-quit share of the waves has run out of work, which
+// shortens turns by itself when redexes are scarce.
 __host__ __device__ __forceinline__ u32 plain_action(u32 n, u32 nwaves) {
   (void)n;
   (void)nwaves;
-  return 1;
+  return 0;
 }
+
+// The steady rule: after a turn that ends with at most NUM/DEN redexes per
+// wave and at most GROWTH_PCT% of those it started with, the next turn runs
+// without idle quit (waves run their chains until they run dry); idle quit
+// comes back inside the turn once a wave holds surplus work.
+#ifndef HVM_STEADY_NUM
+#define HVM_STEADY_NUM 1u
+#endif
+#ifndef HVM_STEADY_DEN
+#define HVM_STEADY_DEN 4u
+#endif
+#ifndef HVM_STEADY_GROWTH_PCT
+#define HVM_STEADY_GROWTH_PCT 150u
+#endif
 
 __device__ __forceinline__ float log2p(float x) { return __log2f(1.0f + x); }
 
@@ -1430,10 +1464,19 @@ __device__ void sched_features(const HvmCtl* c, const HvmSched* s, u32 n_next, u
   f[7] = itrs > 0 ? (float)T.defers / itrs : 0.0f;
   f[8] = fminf(1.0f, (float)T.probes / ws / 4.0f);
   for (int k = 0; k < HVM_K_KINDS; ++k) f[9 + k] = itrs > 0 ? (float)T.kind[k] / itrs : 0.0f;
-  f[19 + (s->last_action & 3)] = 1.0f;
+  // how the turn ended: waves that ran dry, without idle quit (steady), growth
+  f[19] = fminf(1.0f, (float)c->idle_waves / (float)nwaves);
+  f[20] = c->idle_quit == 0 ? 1.0f : 0.0f;
+  f[21] = fminf(1.0f, fmaxf(0.0f, 0.5f + 0.0625f * (log2p((float)n_next) - log2p((float)c->n_in))));
+  f[22] = c->surplus ? 1.0f : 0.0f;
   f[23] = log2p((float)c->turns) / 16.0f;
   f[24] = fminf(1.0f, ws / ((float)nwaves * (float)(c->max_steps ? c->max_steps : 1)));
   f[25] = 1.0f;
+  // per-wave occupancy at the end of the turn, free-slot density of the
+  // allocator's windows, and whether some wave held surplus work
+  for (int k = 0; k < 4; ++k) f[26 + k] = (float)T.wave_left[k] / (float)nwaves;
+  f[30] = T.probes ? fminf(1.0f, (float)T.alloc_slots / (64.0f * (float)T.probes)) : 1.0f;
+  f[31] = log2p(itrs / fmaxf(dt_us, 1.0f)) / 16.0f;
 }
 
 __device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
@@ -1471,12 +1514,12 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
         sched_features(c, s, n, A.nwaves, dt_us, f);
         // few redexes, not growing: independent sequential chains, which a
         // redistribution cannot spread; the next turn runs without idle quit
-        s_steady = n <= A.nwaves / 4 && 2 * n <= 3 * c->n_in;
+        s_steady = (u64)n * HVM_STEADY_DEN <= (u64)A.nwaves * HVM_STEADY_NUM &&
+                   (u64)n * 100u <= (u64)c->n_in * HVM_STEADY_GROWTH_PCT;
         c->n_in = n;
         c->n_out = 0;
         c->idle_waves = 0;
         c->surplus = 0;
-        c->in_head = min(n, A.nwaves * (u32)HVM_WAVE);
         c->parity ^= 1u;
         s_n = n;
         if (n == 0) c->done = 1;
@@ -1523,13 +1566,21 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
   } else if (s->mode == HVM_SCHED_EXPLORE) {
     const u32 h = sched_hash(s->seed, c->turns);
     if (h % 100u < s->explore_pct) action = (h >> 8) & 3u;
+  } else if (s->mode == HVM_SCHED_FIXED) {
+    action = s->explore_pct % HVM_ACTIONS;
   }
   if (t == 0) {
     // a nonzero seed shortens each turn by up to an eighth (share jitter)
-    const u32 len = kTurnLen[action];
+    const u32 len = HVM_TURN_LEN;
     const u32 jit = s->seed ? (sched_hash(s->seed ^ 0xA5A5A5A5u, c->turns) >> 4) % (len / 8u + 1u) : 0u;
     c->max_steps = c->fixed_steps ? c->fixed_steps : len - jit;
-    c->idle_quit = kIdlePct[action] ? max(1u, (A.nwaves * kIdlePct[action]) / 100u) : A.idle_quit;
+    const u32 act = max(1u, A.nwaves / kWaveDiv[action]);
+    // the configured idle-quit count, scaled to the waves that run
+    const u32 back = A.idle_quit ? max(1u, (u32)(((u64)A.idle_quit * act) / A.nwaves)) : 0u;
+    c->active = act;
+    c->in_head = min(n, act * (u32)HVM_WAVE);
+    c->idle_back = back;
+    c->idle_quit = kIdlePct[action] ? max(1u, (act * kIdlePct[action]) / 100u) : back;
     if (s_steady) c->idle_quit = 0;
     s->last_action = action;
     if (s->trace && s->trace_len < s->trace_cap) {
@@ -1566,8 +1617,9 @@ extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, uint32_t clock_kh
 // Turn length of the first turn (the host boots it; later turns are chosen
 // on the device).
 extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves) {
-  static const uint32_t len[HVM_ACTIONS] = {4096, 4096, 4096, 512};
-  return len[plain_action(n, nwaves)];
+  (void)n;
+  (void)nwaves;
+  return HVM_TURN_LEN;
 }
 
 extern "C" uint32_t hvm_plain_action_host(uint32_t n, uint32_t nwaves) { return plain_action(n, nwaves); }

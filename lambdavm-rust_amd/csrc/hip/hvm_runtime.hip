@@ -396,7 +396,7 @@ struct hvm_hip_ctx {
   }
 
   bool set_scheduler(u32 mode, u32 seed, u32 explore_pct, const unsigned short* params, u32 trace_cap) {
-    if (mode > HVM_SCHED_EXPLORE) return fail("unknown scheduler mode");
+    if (mode > HVM_SCHED_FIXED) return fail("unknown scheduler mode");
     if (mode == HVM_SCHED_LEARNED && !params) return fail("learned scheduler needs parameters");
     if (params && !ck(hipMemcpy(d_params, params, HVM_MLP_PARAMS * sizeof(unsigned short), hipMemcpyHostToDevice),
                       "params upload")) return false;
@@ -440,7 +440,7 @@ struct hvm_hip_ctx {
     c.in_head = 1;
     c.parity = 0;
     c.fixed_steps = std::min(cfg.turn_steps, 1u << 20);
-    c.idle_quit = turn_args().idle_quit;
+    c.idle_quit = c.idle_back = turn_args().idle_quit;
     c.max_steps = cfg.turn_steps ? std::min(cfg.turn_steps, 1u << 20) : hvm_turn_policy_host(1, nwaves);
     c.turns = 0;
     c.done = 0;
@@ -1118,7 +1118,7 @@ struct hvm_hip_ctx {
     c.n_in = n;
     c.in_head = std::min<u32>(n, nwaves * HVM_WAVE);
     c.fixed_steps = std::min(cfg.turn_steps, 1u << 20);
-    c.idle_quit = turn_args().idle_quit;
+    c.idle_quit = c.idle_back = turn_args().idle_quit;
     c.max_steps = cfg.turn_steps ? std::min(cfg.turn_steps, 1u << 20) : hvm_turn_policy_host(n, nwaves);
     host_ctl = c;
     if (!ck(hipMemcpyAsync(ctl, &host_ctl, sizeof(HvmCtl), hipMemcpyHostToDevice, stream), "boot ctl")) return false;

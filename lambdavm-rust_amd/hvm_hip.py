@@ -15,7 +15,9 @@ from dataclasses import dataclass, field
 from typing import List, Optional
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libhvm_hip.so")
+# HVM_HIP_LIB selects another build of the library, such as a variant with
+# other compile-time knobs (lib/variants/, searched by autotune.py).
+LIB_PATH = os.environ.get("HVM_HIP_LIB") or os.path.join(_HERE, "lib", "libhvm_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hvm_hip.h")
 
 KINDS = ["CALL", "CALL_COPY", "VOID", "ERAS", "ANNI", "COMM", "OPER1", "OPER2", "SWIT0", "SWITN"]
@@ -34,7 +36,7 @@ GRAPHS = 64
 SPEC_NODES = 64
 
 # Scheduler (include/hvm_hip.h)
-SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE = 0, 1, 2
+SCHED_PLAIN, SCHED_LEARNED, SCHED_EXPLORE, SCHED_FIXED = 0, 1, 2, 3
 TURN_LENGTHS = (4096, 4096, 4096, 512)
 IDLE_PCTS = (10, None, 50, None)   # None: the configured idle_pct
 MLP_FEAT, MLP_HID, MLP_ACT = 32, 64, 4

@@ -1,20 +1,31 @@
-"""Learned turn-length scheduler: trace collection, training and the A/B test.
+"""Learned scheduler: trace collection, training and the A/B test.
 
-The evaluator's scheduler chooses, after every turn, how many steps the next
-turn runs before the HBM bag redistributes redexes (one of 24/96/512/4096).
-The plain heuristic looks only at the number of pending redexes.  The learned
+After every turn the evaluator's scheduler chooses how the next turn shares
+work, one of 4 (wave share, idle-quit share) actions: (all, the configured
+25%), (all, 50%), (half, 25%), (all, 10%).  A turn runs on that share of the
+waves and ends once its idle-quit share of them has run dry; then the HBM bag
+redistributes the redexes.  The plain heuristic always takes action 0 (with
+the steady-chain and surplus rules of the turn kernel).  The learned
 scheduler is a 32->64->64->4 MLP (fp16 weights, fp32 accumulation) that reads
 the finished turn's features — pending redexes, lane occupancy, per-kind
-interaction mix, spill/defer/probe rates, the previous choice — and predicts,
-for each turn length, the log2 interactions-per-µs of the turn it would
-govern; it takes the argmax.
+interaction mix, spill/defer/probe rates, how the turn ended (waves run dry,
+steady, growth), per-wave occupancy at its end (waves that left 0, 1-64,
+65-128, >128 redexes), free-slot density, rate — and predicts, for each
+action, log2 of the speedup of the whole evaluation when that action is
+taken; it takes the argmax, and keeps action 0 unless the prediction beats
+it by a margin.
 
-* `collect_traces` runs the plain heuristic with seeded random choices mixed
-  in (explore mode) and records (features, choice, reward) per decision;
-* `train` fits the MLP on those records with the library's HIP training
-  kernels (full-batch Adam);
-* `ab_test` times plain vs learned on other seeds and checks that both give
-  the same normal form and interaction count.
+* `collect_fixed` evaluates a program once per (seed, action) with the action
+  fixed for every turn (fixed mode), recording each turn's features and the
+  evaluation's time;
+* `fixed_targets` turns those runs into training rows: every recorded turn
+  of a program gets, for each action, log2(median time under action 0 /
+  median time under that action) — the reward is the whole evaluation's
+  time, not the next turn's;
+* `train` fits the MLP on those rows with the library's HIP training kernels
+  (full-batch Adam);
+* `ab_test` times plain vs learned and checks that both give the same normal
+  form and interaction count.
 
 The scheduler never changes an interaction, so results are bit-identical
 whichever policy runs; only time changes.
@@ -24,25 +35,45 @@ from __future__ import annotations
 import math
 import random
 import statistics
-from typing import List, Sequence, Tuple
+from typing import Dict, List, Sequence, Tuple
 
 import hvm_hip as H
 
 
-def collect_traces(words, seeds: Sequence[int], explore_pct: int = 35, run=None, **cfg):
-    """Runs the program once per seed in explore mode (`run(ctx)` evaluates
-    it; default ctx.eval()); returns the decision records that have a
-    measured reward."""
-    recs = []
+def collect_fixed(words, seeds: Sequence[int], run=None, **cfg) -> Dict[int, List[Tuple[float, list]]]:
+    """Evaluates the program once per seed and action, the action fixed for
+    every turn (`run(ctx)` evaluates it and returns its time in ms; default
+    ctx.eval()).  Actions alternate within a seed, so that slow drifts of the
+    device hit all of them alike.  Returns {action: [(ms, [features per
+    turn]), ...]}."""
+    run = run or (lambda c: c.eval().eval_ms)
+    out: Dict[int, List[Tuple[float, list]]] = {a: [] for a in range(H.MLP_ACT)}
     with H.Context(flags=H.NO_READBACK, **cfg) as ctx:
         ctx.load(words)
+        run(ctx)
         for s in seeds:
-            ctx.set_scheduler(H.SCHED_EXPLORE, seed=s, explore_pct=explore_pct, trace_cap=1 << 16)
-            (run or (lambda c: c.eval()))(ctx)
-            for feats, action, reward, itrs, us in ctx.read_trace():
-                if us > 0:
-                    recs.append((feats, action, reward))
-    return recs
+            for a in (range(H.MLP_ACT) if s % 2 == 0 else reversed(range(H.MLP_ACT))):
+                ctx.set_scheduler(H.SCHED_FIXED, seed=s, explore_pct=a, trace_cap=1 << 16)
+                ms = run(ctx)
+                out[a].append((ms, [feats for feats, action, reward, itrs, us in ctx.read_trace()]))
+    return out
+
+
+def fixed_targets(runs: Dict[int, List[Tuple[float, list]]],
+                  per_run: int = 64) -> Tuple[List[Tuple[list, int, float]], List[float]]:
+    """Training rows of one program: every turn recorded under any action
+    (at most `per_run` turns of a run, evenly spaced) gets one row per action
+    a, with target log2(T0 / Ta), where Ta is the median evaluation time with
+    a fixed.  Returns (rows, [Ta])."""
+    med = [statistics.median(ms for ms, _ in runs[a]) for a in range(len(runs))]
+    gain = [math.log2(med[0] / m) for m in med]
+    rows = []
+    for a in range(len(runs)):
+        for _, turns in runs[a]:
+            step = max(1.0, len(turns) / per_run)
+            for f in (turns[int(i * step)] for i in range(min(per_run, len(turns)))):
+                rows += [(f, k, gain[k]) for k in range(len(runs))]
+    return rows, med
 
 
 def init_params(seed: int = 0) -> List[float]:

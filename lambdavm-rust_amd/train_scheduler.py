@@ -1,19 +1,26 @@
-"""Trains the learned turn-length scheduler on the GPU and writes its weights.
+"""Trains the learned scheduler on the GPU and writes its weights.
 
     python lambdavm-rust_amd/train_scheduler.py [--n 12 --m 12] [--train-seeds 64]
-        [--also bitonic,radix,sum,batch] [--also-seeds 16]
-        [--eval-runs 16] [--out lambdavm-rust_amd/sched_weights.json]
+        [--also radix_lsd,batch,stress] [--also-seeds 8]
+        [--eval-runs 12] [--out lambdavm-rust_amd/sched_weights.json]
 
-Workload: church(n, m) of programs.py (2^n iterations of the Church numeral
-2^m applied to a u24 increment).  Traces come from the plain heuristic with
-seeded random turn lengths mixed in (seeds 0..train_seeds-1); `--also` adds
-traces of other workloads (bitonic 2^16, radix 2^18, sum 2^20, the
-256-net batch of programs.batch(0); seeds 0..also_seeds-1), so that one
-network serves them all.  The MLP is fitted
-with the library's HIP kernels; then plain and learned schedules are timed
-on alternating runs at held-out seeds (no trace run is reused) on every
-workload, and both must give the same normal form and ITRS.  The weights file
-holds the fp16 parameters (as integers), the reward offset and the report.
+Workload: church_loop(n, m) of programs.py (an outer loop of 2^n iterations,
+each applying the Church numeral 2^m to a u24 increment), evaluated at seeds
+0..train_seeds-1 once under each of the 4 actions held fixed for every turn;
+`--also` adds other workloads the same way at seeds 0..also_seeds-1
+(radix_lsd: the LSD radix sort of 2^18 keys; batch: the 256-net batch of
+programs.batch(0); stress: the reference's stress program with 2^16 loops of
+256; bitonic, radix, sum, church: the earlier sort, sum and Church-fan
+programs), so that one network serves them all.  Every recorded turn of a
+workload is a training input, and its targets are the whole-evaluation
+speedups of the 4 actions on that workload (log2 of median time under
+action 0 over median time under the action).  The MLP is fitted with the
+library's HIP kernels; the margin by which it must beat action 0 is picked on
+training seeds; then plain and learned schedules are timed on alternating
+runs at held-out seeds (which perturb placement and turn lengths in both
+modes) on every workload and on programs the network never saw, and both
+must give the same normal form and ITRS.  The weights file holds the fp16
+parameters (as integers) and the report.
 """
 import argparse
 import json
@@ -36,14 +43,17 @@ def main():
     ap.add_argument("--n", type=int, default=12)
     ap.add_argument("--m", type=int, default=12)
     ap.add_argument("--train-seeds", type=int, default=64)
-    ap.add_argument("--explore-pct", type=int, default=35)
-    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--epochs", type=int, default=800)
     ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--eval-runs", type=int, default=16)
-    ap.add_argument("--also", default="", help="more workloads to train on: bitonic,radix,sum,batch")
-    ap.add_argument("--also-seeds", type=int, default=16)
-    ap.add_argument("--margins", type=lambda v: [int(x) for x in v.split(",")], default=[0, 5, 10, 20, 40],
-                    help="candidate margins (hundredths of log2 rate) for the plain fallback")
+    ap.add_argument("--eval-runs", type=int, default=12)
+    ap.add_argument("--also", default="radix_lsd,batch,stress",
+                    help="more workloads to train on: radix_lsd,batch,stress,bitonic,radix,sum,church")
+    ap.add_argument("--also-seeds", type=int, default=8)
+    ap.add_argument("--unseen", default="church_loop_11,stress_14",
+                    help="programs evaluated but never trained on: church_loop_11 (church_loop(11, 12)), "
+                         "stress_14 (stress with 2^14 loops of 256)")
+    ap.add_argument("--margins", type=lambda v: [int(x) for x in v.split(",")], default=[0, 2, 4, 8],
+                    help="candidate margins (hundredths of log2 speedup) for the plain fallback")
     ap.add_argument("--out", default=os.path.join(HERE, "sched_weights.json"))
     args = ap.parse_args()
 
@@ -58,80 +68,99 @@ def main():
             return tot.eval_ms, (tot.itrs, tuple(p.canonical for p in per)), None
         return run
 
-    church = P.church(args.n, args.m)
-    progs = {f"church(n={args.n}, m={args.m})": (church[0], str(church[1]), args.train_seeds, single, {})}
-    extra = {"bitonic": ("bitonic(16, 0)", lambda: P.bitonic(16, 0)),
-             "radix": ("radix(18, 0)", lambda: P.radix(18, 0)),
-             "sum": ("sum_tree(20)", lambda: P.sum_tree(20))}
-    for name in [x for x in args.also.split(",") if x]:
+    def program(name):
+        """(label, code, expected result or None, run, context config)"""
         if name == "batch":
             code, nets = P.batch(0, 256)
             fids = P.def_fids(code)
             entries = [fids[f"net{i}"] for i in range(len(nets))]
             est = [P.batch_estimate(gen, params) for gen, params, _, _ in nets]
-            progs["batch(0, 256 nets)"] = (code, None, args.also_seeds, batch_run(entries, est),
-                                           {"node_cap": 1 << 29, "vars_cap": 1 << 29})
-            continue
-        label, gen = extra[name]
+            return "batch(0, 256 nets)", code, None, batch_run(entries, est), {"node_cap": 1 << 29, "vars_cap": 1 << 29}
+        gens = {"bitonic": ("bitonic(16, 0)", lambda: P.bitonic(16, 0)),
+                "radix": ("radix(18, 0)", lambda: P.radix(18, 0)),
+                "radix_lsd": ("radix_lsd(18, 0)", lambda: P.radix_lsd(18, 0)),
+                "sum": ("sum_tree(20)", lambda: P.sum_tree(20)),
+                "church": ("church(12, 12)", lambda: P.church(12, 12)),
+                "stress": ("stress(16, 256)", lambda: P.stress(16, 256)),
+                "stress_14": ("stress(14, 256)", lambda: P.stress(14, 256)),
+                "church_loop_11": ("church_loop(11, 12)", lambda: P.church_loop(11, 12))}
+        label, gen = gens[name]
         code, expect = gen()
-        progs[label] = (code, None, args.also_seeds, single, {})
-    books = {label: H.compile_book(v[0]) for label, v in progs.items()}
+        # the sorts print trees; the others a number
+        return label, code, None if name in ("bitonic", "radix") else str(expect), single, {}
+
+    church = P.church_loop(args.n, args.m)
+    progs = {f"church_loop(n={args.n}, m={args.m})": (church[0], str(church[1]), args.train_seeds, single, {})}
+    for name in [x for x in args.also.split(",") if x]:
+        label, code, expect, run, cfg = program(name)
+        progs[label] = (code, expect, args.also_seeds, run, cfg)
+    unseen = {}
+    for name in [x for x in args.unseen.split(",") if x]:
+        label, code, expect, run, cfg = program(name)
+        unseen[label] = (code, expect, 0, run, cfg)
+    books = {label: H.compile_book(v[0]) for label, v in list(progs.items()) + list(unseen.items())}
 
     t0 = time.time()
-    recs = []
+    rows, fixed_ms = [], {}
     for label, (_, _, nseeds, run, cfg) in progs.items():
-        r = S.collect_traces(books[label], range(nseeds), args.explore_pct, run=run, time_limit_s=300, **cfg)
-        print(f"traces: {label}: {len(r)} decisions from {nseeds} runs", flush=True)
-        recs += r
-    print(f"traces: {len(recs)} decisions in {time.time() - t0:.1f}s", flush=True)
-    p16, p32, loss, off = S.train(recs, epochs=args.epochs, lr=args.lr)
-    print(f"trained: loss {loss:.4f} (reward offset {off:.3f})", flush=True)
+        runs = S.collect_fixed(books[label], range(nseeds), run=lambda c, run=run: run(c)[0], time_limit_s=300, **cfg)
+        r, med = S.fixed_targets(runs)
+        fixed_ms[label] = med
+        print(f"fixed: {label}: median ms per action {[round(m, 2) for m in med]} ({len(r)} rows, {nseeds} seeds)",
+              flush=True)
+        rows += r
+    print(f"fixed: {len(rows)} rows in {time.time() - t0:.1f}s", flush=True)
+    p16, p32, loss, off = S.train(rows, epochs=args.epochs, lr=args.lr)
+    print(f"trained: loss {loss:.5f} (target offset {off:.4f})", flush=True)
 
-    # The margin by which the network must beat the plain heuristic's choice
-    # (hundredths of log2 rate) is chosen on training seeds: geometric mean of
-    # plain/learned time over the workloads.
-    def ratio(label, seeds, margin):
-        pt, lt = [], []
-        run, cfg = progs[label][3], progs[label][4]
-        with H.Context(time_limit_s=300, flags=H.NO_READBACK, **cfg) as ctx:
+    def ab(label, spec, seeds, margin, check=False):
+        """Plain and learned on alternating runs; returns (plain ms list, learned ms list, identical results)."""
+        code, expect, _, run, cfg = spec
+        plain, learned, keys = [], [], set()
+        with H.Context(time_limit_s=300, flags=0 if check else H.NO_READBACK, **cfg) as ctx:
             ctx.load(books[label])
             run(ctx)
-            for sd in seeds:
-                ctx.set_scheduler(H.SCHED_PLAIN, seed=sd)
-                pt.append(run(ctx)[0])
-                ctx.set_scheduler(H.SCHED_LEARNED, seed=sd, explore_pct=margin, params_fp16=p16)
-                lt.append(run(ctx)[0])
-        return statistics.median(pt) / statistics.median(lt)
+            for i, sd in enumerate(seeds):
+                for mode in ((H.SCHED_PLAIN, H.SCHED_LEARNED) if i % 2 == 0 else (H.SCHED_LEARNED, H.SCHED_PLAIN)):
+                    ctx.set_scheduler(mode, seed=sd, explore_pct=margin if mode == H.SCHED_LEARNED else 0,
+                                      params_fp16=p16 if mode == H.SCHED_LEARNED else None)
+                    ms, key, result = run(ctx)
+                    (plain if mode == H.SCHED_PLAIN else learned).append(ms)
+                    if check:
+                        keys.add(key)
+                        if expect is not None and result != expect:
+                            raise SystemExit(f"{label}: wrong result {result!r}, expected {expect}")
+        return plain, learned, len(keys) <= 1
+
+    # The margin by which the network must beat action 0 (hundredths of log2
+    # speedup) is chosen on training seeds: geometric mean of plain/learned
+    # median time over the workloads.
     margins = {}
     for m in args.margins:
-        rs = [ratio(label, range(4), m) for label in progs]
+        rs = []
+        for label, spec in progs.items():
+            pt, lt, _ = ab(label, spec, range(4), m)
+            rs.append(statistics.median(pt) / statistics.median(lt))
         margins[m] = math.exp(statistics.fmean(math.log(r) for r in rs))
         print(f"margin {m}: geo-mean speedup on training seeds {margins[m]:.4f} {[round(r, 3) for r in rs]}", flush=True)
     margin = max(margins, key=margins.get)
 
     evals = {}
-    for label, (code, expect, nseeds, run, cfg) in progs.items():
-        plain, learned, results = [], [], set()
-        with H.Context(time_limit_s=300, **cfg) as ctx:
-            ctx.load(books[label])
-            for i in range(args.eval_runs):
-                seed = nseeds + i  # held out: no trace run used it
-                for mode in ((H.SCHED_PLAIN, H.SCHED_LEARNED) if i % 2 == 0 else (H.SCHED_LEARNED, H.SCHED_PLAIN)):
-                    ctx.set_scheduler(mode, seed=seed, explore_pct=margin if mode == H.SCHED_LEARNED else 0,
-                                      params_fp16=p16 if mode == H.SCHED_LEARNED else None)
-                    ms, key, result = run(ctx)
-                    (plain if mode == H.SCHED_PLAIN else learned).append(ms)
-                    results.add(key)
-                    if expect is not None and result != expect:
-                        raise SystemExit(f"wrong result {result!r}, expected {expect}")
+    for label, spec in list(progs.items()) + list(unseen.items()):
+        first = spec[2]  # held out: no training run used these seeds
+        seeds = range(first, first + args.eval_runs)
+        plain, learned, same = ab(label, spec, seeds, margin, check=True)
+        if not same:
+            raise SystemExit(f"{label}: plain and learned results differ")
         evals[label] = {"plain_ms_median": statistics.median(plain),
                         "learned_ms_median": statistics.median(learned),
                         "speedup": statistics.median(plain) / statistics.median(learned),
-                        "identical_results": len(results) == 1,
-                        "eval_seeds": [nseeds, nseeds + args.eval_runs - 1]}
+                        "identical_results": same, "trained_on": label in progs,
+                        "eval_seeds": [first, first + args.eval_runs - 1],
+                        "fixed_action_ms": fixed_ms.get(label)}
         print(label, json.dumps(evals[label]), flush=True)
     first = next(iter(progs))
-    report = dict(evals[first], workload=first, train_decisions=len(recs), train_seeds=args.train_seeds,
+    report = dict(evals[first], workload=first, train_rows=len(rows), train_seeds=args.train_seeds,
                   loss=loss, eval_runs=args.eval_runs, trained_on=list(progs), per_workload=evals,
                   margin_pct=margin, margin_search={str(k): v for k, v in margins.items()})
     print(json.dumps({k: v for k, v in report.items() if k != "per_workload"}), flush=True)

@@ -1,6 +1,7 @@
 """CPU checks of the scheduler MLP's numpy reference (the checker of the HIP
 kernels): its analytic gradient agrees with central finite differences, and a
-zero-reward, zero-weight network has zero loss and gradient."""
+zero-reward, zero-weight network has zero loss and gradient; and of the
+training targets built from fixed-action runs."""
 import numpy as np
 
 import mlp_ref
@@ -39,3 +40,26 @@ def test_zero_network():
     X = np.ones((4, mlp_ref.F), dtype=np.float32)
     loss, g = mlp_ref.loss_and_grad(p, X, [0, 1, 2, 3], [0, 0, 0, 0])
     assert loss == 0.0 and not g.any()
+
+
+def test_targets_are_whole_evaluation_speedups():
+    """Every recorded turn of a program, under whichever fixed action it was
+    recorded, gets one row per action: log2 of the median evaluation time
+    under action 0 over the median time under that action; at most
+    `per_run` evenly spaced turns of a run are used."""
+    import math
+
+    import sched as S
+    f = [[float(i)] * 32 for i in range(10)]
+    runs = {0: [(100.0, f[:2]), (120.0, f[2:3]), (80.0, [])],
+            1: [(50.0, f[3:4]), (50.0, [])],
+            2: [(200.0, f[4:10])],
+            3: [(100.0, [])]}
+    rows, med = S.fixed_targets(runs, per_run=3)
+    assert med == [100.0, 50.0, 200.0, 100.0]
+    # turns used: f0 f1 (run 0), f2, f3, and 3 of f4..f9 evenly spaced: f4 f6 f8
+    assert [r[0][0] for r in rows[::4]] == [0.0, 1.0, 2.0, 3.0, 4.0, 6.0, 8.0]
+    assert [r[1] for r in rows[:4]] == [0, 1, 2, 3]
+    assert [r[2] for r in rows[:4]] == [0.0, 1.0, -1.0, 0.0]
+    assert all(r[2] == [0.0, 1.0, -1.0, 0.0][r[1]] for r in rows)
+    assert math.isclose(sum(r[2] for r in rows), 0.0, abs_tol=1e-12)
