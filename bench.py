@@ -135,6 +135,40 @@ def golden_case(golden, gen, params):
     return next((c for c in golden["workloads"] if c["generator"] == gen and c["params"] == params), None)
 
 
+def tuned_run(name, golden):
+    """One autotuned configuration (lambdavm-rust_amd/tuned_<name>.json),
+    evaluated by `autotune.py --child` in a child process that loads the
+    configuration's build: MIPS and ms per evaluation, and whether ITRS and
+    normal form(s) equal the run-c records; None without the file."""
+    import subprocess
+    path = os.path.join(ROOT, "lambdavm-rust_amd", f"tuned_{name}.json")
+    if not os.path.exists(path):
+        return None
+    cfg = json.load(open(path))["config"]
+    env = dict(os.environ)
+    if cfg.get("variant"):
+        env["HVM_HIP_LIB"] = os.path.join(ROOT, "lambdavm-rust_amd", "lib", "variants",
+                                          f"libhvm_hip_{cfg['variant']}.so")
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "lambdavm-rust_amd", "autotune.py"), "--child",
+                        "--workload", name, "--cfg", json.dumps(cfg), "--repeats", "3", "--readback",
+                        "--time-limit", "300"], capture_output=True, text=True, env=env, timeout=900)
+    res = json.loads(p.stdout.strip().splitlines()[-1])
+    if "error" in res:
+        raise RuntimeError(f"tuned configuration of {name} failed: {res['error']}")
+    if name == "batch":
+        gb = json.load(open(os.path.join(ROOT, "tests", "golden", "batch.json")))
+        itrs = gb["total_itrs"]
+        ok = res["ident"][0] == itrs and res["nets_nf_sha256"] == [row["nf_sha256"] for row in gb["nets"]]
+    else:
+        case = next(c for c in golden["workloads"] if c["generator"] == name and c.get("full_size"))
+        itrs = case["itrs"]
+        ok = res["ident"] == [case["itrs"], case["nf_sha256"]]
+    return {"config": cfg, "mips": round(itrs / (res["ms"] * 1e-3) / 1e6, 2), "ms_per_eval": round(res["ms"], 3),
+            "parity": bool(ok), "protocol": "median of 3 evaluations after a warm-up (single nets: readback "
+                                            "included; the batch: timed without it, normal forms read back "
+                                            "from one more evaluation)"}
+
+
 def timed(ctx, steps, warmup, run):
     """W warmup evaluations, then exactly K timed ones (device synchronised
     on both sides: every evaluation ends with the result read back)."""
@@ -152,7 +186,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="radix_lsd18")
-    ap.add_argument("--sched", choices=("plain", "learned"), default="plain",
+    ap.add_argument("--sched", choices=("plain", "learned"), default="learned",
                     help="scheduler whose timed steps are the headline value (both are timed)")
     ap.add_argument("--mode", choices=("interpreted", "compiled"), default="interpreted",
                     help="the book interpreted as run-cu does (default) or with compiled CALLs")
@@ -295,6 +329,20 @@ def main():
         if len(bsched) == 2:
             bsched["learned_over_plain"] = round(bsched["learned"]["mips"] / bsched["plain"]["mips"], 4)
         out["schedulers"]["batch256"] = bsched
+
+        # ---- the autotuner's configurations (lambdavm-rust_amd/tuned_<name>.json:
+        # compiled CALLs, idle-quit share, turn length, graphs, compile-time
+        # variant), each measured by the autotuner's own protocol (median of 3
+        # evaluations after one warm-up) in a child process that loads that
+        # build, ITRS and NF checked against the run-c records ----
+        tuned = {}
+        for key, tname in (("radix_lsd18", "radix_lsd"), ("batch256", "batch")):
+            if key != "batch256" and key != args.workload:
+                continue
+            t = tuned_run(tname, golden)
+            if t is not None:
+                tuned[key] = t
+        out["tuned"] = tuned
 
         # ---- the other sort workloads, after the timed regions ----
         extra = {}

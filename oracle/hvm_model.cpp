@@ -87,6 +87,8 @@ typedef struct hvm_model_out {
 #define HVM_MODEL_TAKE_BEFORE_FIT 8u // every lane with a take rule takes (by exchange) before knowing
                                   // whether its step fits the tables; lanes that then wait put
                                   // their nodes back
+#define HVM_MODEL_XCHG_ONE 16u   // single-take rules and a CALL's copy take by exchange (else by
+                                  // load, zeroed at commit)
 
 int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_t* entries, uint32_t n_nets,
                   const hvm_model_cfg* cfg, hvm_model_out* out);
@@ -550,8 +552,12 @@ void Model::turn(u32 w) {
       x.run = x.has && x.rule != HVM_CALL && x.off_n + x.nn <= TB && x.off_v + x.nv <= TB;
     }
     const bool before_fit = (C.variant & HVM_MODEL_TAKE_BEFORE_FIT) != 0;
-    const bool peek_two = (C.variant & HVM_MODEL_XCHG_TWO) == 0 && !before_fit;
     auto two = [&](const St& x) { return x.take_a && x.take_b; };
+    // early takes read their nodes with loads (zeroed at commit) unless a
+    // variant takes them by exchange
+    auto peek = [&](const St& x) {
+      return early && !before_fit && (C.variant & (two(x) ? HVM_MODEL_XCHG_TWO : HVM_MODEL_XCHG_ONE)) == 0;
+    };
     // the round trip of the early takes and the allocator's first windows
     std::vector<u64> pre_n(L, 1);
     std::vector<u32> pre_v(L, 1);
@@ -568,8 +574,8 @@ void Model::turn(u32 w) {
       for (u32 op : ops) {
         u32 l = op % L, kind_ = op / L;
         St& x = st[l];
-        if (kind_ == 0) { x.ga = node_gen[nslot(hvm_get_val(x.a))]; x.NA = peek_two && two(x) ? node_load(hvm_get_val(x.a)) : node_xchg(hvm_get_val(x.a), 0); }
-        if (kind_ == 1) { x.gb = node_gen[nslot(hvm_get_val(x.b))]; x.NB = peek_two && two(x) ? node_load(hvm_get_val(x.b)) : node_xchg(hvm_get_val(x.b), 0); }
+        if (kind_ == 0) { x.ga = node_gen[nslot(hvm_get_val(x.a))]; x.NA = peek(x) ? node_load(hvm_get_val(x.a)) : node_xchg(hvm_get_val(x.a), 0); }
+        if (kind_ == 1) { x.gb = node_gen[nslot(hvm_get_val(x.b))]; x.NB = peek(x) ? node_load(hvm_get_val(x.b)) : node_xchg(hvm_get_val(x.b), 0); }
         if (kind_ == 2) { u32 rel = (s.node_cur - n_lo + l) % node_slice; pre_n[l] = node_load(n_lo + rel); }
         if (kind_ == 3) { u32 rel = (s.vars_cur - v_lo + l) % vars_slice; pre_v[l] = vars_load(v_lo + rel); }
       }
@@ -629,7 +635,7 @@ void Model::turn(u32 w) {
         for (u32 op : ops) {
           St& x = st[op % L];
           x.gb = node_gen[nslot(hvm_get_val(x.b))];
-          x.NB = node_xchg(hvm_get_val(x.b), 0);
+          x.NB = peek(x) ? node_load(hvm_get_val(x.b)) : node_xchg(hvm_get_val(x.b), 0);
         }
         ops.clear();
       }
@@ -660,7 +666,7 @@ void Model::turn(u32 w) {
       if ((x.take_a && x.NA == 0) || (x.take_b && x.NB == 0)) {
         O->take_zero++;
         event(3);
-        const bool peeked = peek_two && two(x);
+        const bool peeked = peek(x);
         if (!peeked && x.take_a && x.NA) {
           O->putback++;
           if (node_gen[nslot(hvm_get_val(x.a))] != x.ga) { O->clobber++; event(5); }
@@ -680,7 +686,10 @@ void Model::turn(u32 w) {
     // peeked nodes are taken now
     for (u32 l : order(L)) {
       St& x = st[l];
-      if (x.run && peek_two && two(x)) { node_store(hvm_get_val(x.a), 0); node_store(hvm_get_val(x.b), 0); }
+      if (x.run && peek(x)) {
+        if (x.take_a) node_store(hvm_get_val(x.a), 0);
+        if (x.take_b) node_store(hvm_get_val(x.b), 0);
+      }
     }
     // ---- P4: creation ----
     for (u32 l : order(L)) {

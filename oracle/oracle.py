@@ -152,6 +152,7 @@ MODEL_XCHG_TWO = 1
 MODEL_LATE_TAKES = 2
 MODEL_OVERLAP_SCAN = 4
 MODEL_TAKE_BEFORE_FIT = 8
+MODEL_XCHG_ONE = 16
 
 
 class _ModelCfg(C.Structure):

@@ -79,10 +79,12 @@ def test_batch_idle_off_is_clean(batch, orc, seed):
     assert all(result_ok(t, e) for t, (_, e) in zip(r.results, nets))
 
 
-def test_batch_idle_off_exchange_takes_is_clean(batch, orc):
-    # the earlier take mode of ANNI/COMM (exchanges, put back on a retry)
+@pytest.mark.parametrize("variant", ["MODEL_XCHG_TWO", "MODEL_XCHG_ONE"])
+def test_batch_idle_off_exchange_takes_is_clean(batch, orc, variant):
+    # the earlier take modes (exchanges, put back on a retry): of ANNI/COMM,
+    # and of single-take rules and a CALL's copy (HVM_PEEK_TAKES=0)
     words, entries, nets, itrs = batch
-    r = orc.run_model(words, entries, seed=3, variant=orc.MODEL_XCHG_TWO, **CFG)
+    r = orc.run_model(words, entries, seed=3, variant=getattr(orc, variant), **CFG)
     assert r.err == 0 and r.itrs == itrs
     assert r.events["take_zero"] == 0
 
