@@ -175,6 +175,8 @@ struct HvmSched {
   unsigned long long last_clock; // wall clock (100 MHz) at the previous decision
   const unsigned short* params;  // HVM_MLP_PARAMS fp16 values (learned mode)
   float* trace;                  // trace_cap * HVM_TRACE_REC floats, or null
+  float qavg[HVM_ACTIONS];       // learned mode: the network's outputs averaged over the turns
+  uint32_t qn;                   // of this normalisation (exponentially), and their count
   HvmTurnStats turn;
 };
 

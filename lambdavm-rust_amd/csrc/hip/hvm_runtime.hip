@@ -391,6 +391,8 @@ struct hvm_hip_ctx {
     s.last_action = hvm_plain_action_host(n_first, nwaves);
     s.params = d_params;
     s.trace = d_trace;
+    memset(s.qavg, 0, sizeof s.qavg);
+    s.qn = 0;
     memset(&s.turn, 0, sizeof s.turn);
     return ck(hipMemcpy(sched, &s, sizeof s, hipMemcpyHostToDevice), "sched upload");
   }

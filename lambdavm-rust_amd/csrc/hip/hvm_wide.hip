@@ -251,8 +251,20 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
       break;
     }
     if ((step & 15) == 15) {
+      // a turn that started without idle quit (few redexes, not growing) gets
+      // it back once some wave holds more than two redexes per lane: work
+      // grew inside the turn, and only the end of the turn spreads it
+      u32 iq = idle_quit;
+      if (!iq && C->idle_base) {
+        u32 sur = __hip_atomic_load(&C->surplus, __ATOMIC_RELAXED, AGENT);
+        if (!sur && wave_sum64(ln.top - ln.bot) > 2u * HVM_WAVE) {
+          if (lane == 0) __hip_atomic_store(&C->surplus, 1u, __ATOMIC_RELAXED, AGENT);
+          sur = 1;
+        }
+        if (sur) iq = C->idle_base;
+      }
       const u32 idle = __hip_atomic_load(&C->idle_waves, __ATOMIC_RELAXED, AGENT);
-      if (idle_quit && idle >= idle_quit) break;
+      if (iq && idle >= iq) break;
       if (__hip_atomic_load(&C->err, __ATOMIC_RELAXED, AGENT) != 0) break;
     }
     steps++;
@@ -511,6 +523,7 @@ __global__ void hvm_wide_between_kernel(HvmWideArgs A) {
   C->parity ^= 1u;
   C->turns += 1;
   C->idle_waves = 0;
+  C->surplus = 0;
   if (C->n_in == 0 || C->err) C->done = 1;
 }
 

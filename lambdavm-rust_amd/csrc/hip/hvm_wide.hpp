@@ -26,7 +26,9 @@
 struct HvmWideCtl {
   uint32_t n_in, n_out, in_head, err;
   uint32_t parity, max_steps, turns, done;
-  uint32_t idle_waves, idle_quit, idle_base, pad[5];
+  uint32_t idle_waves, idle_quit, idle_base;
+  uint32_t surplus;   // some wave of the current turn held more redexes than two per lane
+  uint32_t pad[4];
 };
 
 struct HvmWideArgs {
