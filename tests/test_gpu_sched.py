@@ -142,17 +142,17 @@ def _ab(hip, ctx, run, seeds, w):
 
 def test_trained_scheduler_gain(hip):
     """The committed scheduler weights (lambdavm-rust_amd/sched_weights.json:
-    trained on church_loop(12, 12) at seeds 0-63 and on the LSD sort, the
-    256-net batch and stress at seeds 0-7, each under every action held
-    fixed) against the plain heuristic at held-out seeds 100-109, with
-    identical normal form and ITRS in every run:
+    trained on church_loop(12, 12) at seeds 0-63 and on seven other workloads
+    at seeds 0-7, each under every action held fixed) against the plain
+    heuristic at held-out seeds 100-109, with identical normal form and ITRS
+    in every run:
     * church_loop(12, 12), the outer-loop Church workload: learned at least 3%
-      faster (median time; measured 5.5% by train_scheduler.py at seeds
-      64-75: it runs these turns on half of the waves, or with a 50%
-      idle-quit share);
+      faster (median time; measured 1.094x by train_scheduler.py at seeds
+      64-75: it runs its turns on half of the waves, or with a 50% idle-quit
+      share, each worth 1.10x when held fixed);
     * the 256-net batch: learned not slower than plain by more than 0.5%
-      (measured 1.1% faster at seeds 8-19; its only better action, idle quit
-      at 10%, is worth 1% when held fixed)."""
+      (measured 1.000x at seeds 8-19: no action held fixed beats plain by
+      more than 1% there, and the fallback margin keeps plain)."""
     path = os.path.join(ROOT, "lambdavm-rust_amd", "sched_weights.json")
     w = json.load(open(path))
     g = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
