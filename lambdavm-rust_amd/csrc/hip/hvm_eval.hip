@@ -79,6 +79,16 @@ __device__ __forceinline__ void vars_put(const HvmTurnArgs& A, u32 val, u32 x) {
 // Per-wave LDS state
 // ---------------------------------------------------------------------------
 
+// HBM evaluator: node slots that the wave's interactions consumed are kept
+// on a per-wave LDS stack of up to HVM_NODE_RECYCLE entries, not zeroed, and
+// the wave's next allocations take them first; what is left is zeroed (freed)
+// at the end of the turn.  A consumed node is named by nothing else, and its
+// slot stays non-zero, so no allocator hands it out meanwhile.  0 zeroes every
+// consumed slot at once.
+#ifndef HVM_NODE_RECYCLE
+#define HVM_NODE_RECYCLE 256
+#endif
+
 struct WaveLds {
   u64 deq[HVM_DEQ * HVM_WAVE];  // deque entry j of lane l at [j*64 + l]
   u32 tbl_n[HVM_TBL_NODE + HVM_TBL_SLACK];
@@ -86,6 +96,9 @@ struct WaveLds {
   u64 inbox[HVM_WAVE];
   u32 target[HVM_WAVE];
   u32 cursor_n, cursor_v;       // allocator scratch
+#if HVM_NODE_RECYCLE
+  u32 rec[HVM_NODE_RECYCLE];    // consumed node slots, not yet zeroed
+#endif
 };
 
 struct Lane {
@@ -423,11 +436,22 @@ __device__ __forceinline__ bool scan_advance(Scan& s, u32 pool_lo, u32 pool_len,
 // both buffers are scanned concurrently.  All arguments are wave-uniform.
 // With `pre`, pre_n/pre_v are this lane's values of the first window of each
 // buffer, loaded earlier at the same cursor.
+// With rec_n > 0 the first node slots come from the wave's recycled ones.
 __device__ __forceinline__ bool wave_alloc2(WaveLds& L, const HvmTurnArgs& A, u32 n_lo, u32 v_lo,
                                             HvmWaveState& cur, u32 need_n, u32 need_v, u64& probes, u32& oom,
-                                            u32* tbl_n, u32* tbl_v, bool pre, u64 pre_n, u32 pre_v) {
+                                            u32* tbl_n, u32* tbl_v, bool pre, u64 pre_n, u32 pre_v,
+                                            u32& rec_n) {
   const u32 lane = lane_id();
-  Scan sn = {n_lo, A.node_slice, cur.node_cur, 0, need_n, 0, false, cur.node_cur};
+  u32 got_n = 0;
+#if HVM_NODE_RECYCLE
+  if (rec_n && need_n) {
+    got_n = min(need_n, rec_n);
+    __syncthreads();
+    for (u32 i = lane; i < got_n; i += HVM_WAVE) tbl_n[i] = L.rec[rec_n - got_n + i];
+    rec_n -= got_n;
+  }
+#endif
+  Scan sn = {n_lo, A.node_slice, cur.node_cur, got_n, need_n, 0, false, cur.node_cur};
   Scan sv = {v_lo, A.vars_slice, cur.vars_cur, 0, need_v, 0, false, cur.vars_cur};
   L.cursor_n = cur.node_cur;
   L.cursor_v = cur.vars_cur;
@@ -564,6 +588,10 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   ln.voids = 0;
   u64 probes = 0, steps = 0, defers = 0, retries = 0, lane_steps = 0, alloc_slots = 0;
   u32 left_tot = 0;   // redexes this wave leaves to the output bag
+  // node slots consumed this turn and not yet handed out again or freed
+  // (wave-uniform; HBM evaluator with early takes by load only)
+  constexpr bool RECYCLE = !LOC && HVM_EARLY_TAKES && HVM_PEEK_TAKES && HVM_NODE_RECYCLE > 0;
+  u32 rec_n = 0;
 
   if (LOC) {
     // an empty local net: every slot free
@@ -885,7 +913,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       __syncthreads();
       if (need_xn || need_xv) {
         u32 oom = 0;
-        if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_xn, need_xv, probes, oom, N->xtn, N->xtv, false, 0, 0)) {
+        if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_xn, need_xv, probes, oom, N->xtn, N->xtv, false, 0, 0, rec_n)) {
           atomicOr(&A.ctl->err, oom);
           if (run) ln.top++;
           break;
@@ -893,7 +921,8 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       }
     } else if (need_n || need_v) {
       u32 oom = 0;
-      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom, L.tbl_n, L.tbl_v, true, pre_n, pre_v)) {
+      if (!wave_alloc2(L, A, n_lo, v_lo, ws, need_n, need_v, probes, oom, L.tbl_n, L.tbl_v, true, pre_n, pre_v,
+                       rec_n)) {
         // out of slots in the slice and the pool: the evaluation stops with an error
         atomicOr(&A.ctl->err, oom);
         if (run) ln.top++;
@@ -924,8 +953,23 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       run = false;
       retries++;
     }
-    // nodes read by a peek are taken now: their slots are zeroed
-    if (PEEK && run) {
+    // nodes read by a peek are taken now: their slots go on the wave's
+    // recycling stack (zeroed, i.e. freed, when it is full)
+    if (RECYCLE) {
+      const bool fa = run && take_a && hvm_get_val(a) < A.node_cap;
+      const bool fb = run && take_b && hvm_get_val(b) < A.node_cap;
+      u32 tot;
+      u32 k = wave_scan((u32)fa + (u32)fb, &tot);
+      const u32 room = HVM_NODE_RECYCLE - rec_n;
+#if HVM_NODE_RECYCLE
+      if (fa) { if (k < room) L.rec[rec_n + k] = hvm_get_val(a); else node_put(A, hvm_get_val(a), 0ull); ++k; }
+      if (fb) { if (k < room) L.rec[rec_n + k] = hvm_get_val(b); else node_put(A, hvm_get_val(b), 0ull); }
+#endif
+      rec_n += min(tot, room);
+      // a take through a port outside the net is zeroed as before
+      if (run && take_a && !fa) node_put(A, hvm_get_val(a), 0ull);
+      if (run && take_b && !fb) node_put(A, hvm_get_val(b), 0ull);
+    } else if (PEEK && run) {
       if (take_a) node_put(A, hvm_get_val(a), 0ull);
       if (take_b) node_put(A, hvm_get_val(b), 0ull);
     }
@@ -1233,6 +1277,15 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     HVM_PHASE(4);
   }
 #undef HVM_PHASE
+
+  // ---- end of turn: recycled node slots not handed out are freed ----
+#if HVM_NODE_RECYCLE
+  if (RECYCLE && rec_n) {
+    __syncthreads();
+    for (u32 i = lane; i < rec_n; i += HVM_WAVE) node_put(A, L.rec[i], 0ull);
+    rec_n = 0;
+  }
+#endif
 
   // ---- end of turn: everything left goes to the output bag ----
   {

@@ -67,7 +67,7 @@ typedef struct hvm_model_out {
   uint64_t take_zero;      // a take found 0 (retry)
   uint64_t putback;        // nodes put back after a retry
   uint64_t clobber;        // a put-back overwrote a node created after the take
-  uint64_t double_alloc;   // a slot handed out while still in use
+  uint64_t double_alloc;   // a node slot handed out while it holds a live node
   uint64_t max_rounds;     // longest link batch, in rounds
   uint64_t full_scans;     // allocations that moved on from the wave's slice to the shared pool
   uint64_t pool_claims;    // slots claimed in the shared pool
@@ -89,6 +89,10 @@ typedef struct hvm_model_out {
                                   // their nodes back
 #define HVM_MODEL_XCHG_ONE 16u   // single-take rules and a CALL's copy take by exchange (else by
                                   // load, zeroed at commit)
+#define HVM_MODEL_NO_RECYCLE 32u // slots of nodes taken by load are zeroed at commit (else they go
+                                  // on the wave's recycling stack, as HVM_NODE_RECYCLE does)
+#define HVM_MODEL_RECYCLE_KEEP 64u // a planted mistake: slots handed out from the recycling stack
+                                  // stay on it (for the test that the double-allocation check works)
 
 int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_t* entries, uint32_t n_nets,
                   const hvm_model_cfg* cfg, hvm_model_out* out);
@@ -149,6 +153,9 @@ struct Model {
   // slot generation: incremented whenever a slot is handed out, to detect
   // a put-back that overwrites a node created after the take
   std::vector<u32> node_gen;
+  // a node slot holds a live node: set by a store of a node, cleared when the
+  // node is taken (a zero store, an exchange, or a push on a recycling stack)
+  std::vector<uint8_t> live;
   std::vector<u64> bag[2];
   // control block
   u32 n_in = 0, n_out = 0, in_head = 0, err = 0, parity = 0, max_steps = 0, idle_waves = 0, idle_quit = 0;
@@ -184,9 +191,13 @@ struct Model {
   // ---- net accessors (every one is a point where another wave may run) ----
   u32 nslot(u32 loc) const { return std::min(loc, C.node_cap); }
   u32 vslot(u32 val) const { return std::min(val == 0x1FFFFFFFu ? 0u : val, C.vars_cap); }
-  u64 node_xchg(u32 loc, u64 v) { yield(); u64 o = node[nslot(loc)]; node[nslot(loc)] = v; return o; }
+  u64 node_xchg(u32 loc, u64 v) { yield(); u64 o = node[nslot(loc)]; node[nslot(loc)] = v; live[nslot(loc)] = v != 0; return o; }
   u64 node_load(u32 loc) { yield(); return node[nslot(loc)]; }
-  void node_store(u32 loc, u64 v) { yield(); node[nslot(loc)] = v; }
+  void node_store(u32 loc, u64 v) { yield(); node[nslot(loc)] = v; live[nslot(loc)] = v != 0; }
+  void handed_out(u32 slot) {
+    node_gen[slot]++;
+    if (live[slot]) { O->double_alloc++; event(6); }
+  }
   u32 vars_xchg(u32 val, u32 x) { yield(); u32 o = vars[vslot(val)]; vars[vslot(val)] = x; vtrace(1, vslot(val), o, x); return o; }
   u32 vars_load(u32 val) { yield(); return vars[vslot(val)]; }
   void vars_store(u32 val, u32 x) { yield(); vtrace(2, vslot(val), vars[vslot(val)], x); vars[vslot(val)] = x; }
@@ -210,7 +221,8 @@ struct Model {
   void link_lanes(std::vector<std::vector<std::pair<u32, u32>>>& batches, std::vector<LaneSt>& lanes,
                   std::vector<u32>& voids, u64& spills);
   bool wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::vector<u32>& tn, std::vector<u32>& tv,
-                  bool pre, const std::vector<u64>& pre_n, const std::vector<u32>& pre_v, u32& oom);
+                  bool pre, const std::vector<u64>& pre_n, const std::vector<u32>& pre_v, u32& oom,
+                  std::vector<u32>& rec);
   void turn(u32 w);
   bool run_turn();
 };
@@ -338,13 +350,21 @@ void Model::link_lanes(std::vector<std::vector<std::pair<u32, u32>>>& batches, s
 // `lanes` slots, own slice first (zero = free), then the shared pool (claim
 // by compare-and-swap).
 bool Model::wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::vector<u32>& tn, std::vector<u32>& tv,
-                       bool pre, const std::vector<u64>& pre_n, const std::vector<u32>& pre_v, u32& oom) {
+                       bool pre, const std::vector<u64>& pre_n, const std::vector<u32>& pre_v, u32& oom,
+                       std::vector<u32>& rec) {
   struct Scan { u32 lo, len, cur, got, need, scanned; bool pool; u32 cursor, slice_next; };
   const u32 L = C.lanes;
   const bool fix_scan = !(C.variant & HVM_MODEL_OVERLAP_SCAN);
   const u32 slice_windows = (C.variant & HVM_MODEL_OVERLAP_SCAN) ? 0xFFFFFFu : 8u;
   WaveState& cs = ws[w];
-  Scan sn = {n_lo, node_slice, cs.node_cur, 0, need_n, 0, false, cs.node_cur, cs.node_cur};
+  // the wave's recycled node slots first (the top of its stack)
+  const u32 got_n = std::min<u32>(need_n, (u32)rec.size());
+  for (u32 i = 0; i < got_n; ++i) {
+    tn[i] = rec[rec.size() - got_n + i];
+    handed_out(tn[i]);
+  }
+  if (!(C.variant & HVM_MODEL_RECYCLE_KEEP)) rec.resize(rec.size() - got_n);
+  Scan sn = {n_lo, node_slice, cs.node_cur, got_n, need_n, 0, false, cs.node_cur, cs.node_cur};
   Scan sv = {v_lo, vars_slice, cs.vars_cur, 0, need_v, 0, false, cs.vars_cur, cs.vars_cur};
   auto idx = [&](Scan& s, u32 lane, u32& rel) { rel = (s.cur - s.lo + lane) % s.len; return s.lo + rel; };
   auto window = [&](auto& buf, Scan& s, std::vector<u32>& tbl, u32 reserved, auto claim, auto* vals) {
@@ -364,7 +384,7 @@ bool Model::wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::v
       if (r < want) {
         tbl[s.got + r] = slot[l];
         if (s.pool) O->pool_claims++;
-        if (&buf == (void*)&node) node_gen[slot[l]]++;
+        if (&buf == (void*)&node) handed_out(slot[l]);
         else vtrace(4, slot[l], (u32)vals[l], s.pool);
         if (r == want - 1) s.cursor = s.lo + ((rel[l] + 1) % s.len);
       } else if (s.pool) {
@@ -454,6 +474,11 @@ void Model::turn(u32 w) {
     s.vars_ovf = vars_pool_lo + (u32)(((u64)(vars_pool_hi - vars_pool_lo) * w) / W);
   const u32 nin = n_in;
   const std::vector<u64>& bag_in = bag[parity];
+  // node slots taken by load and not yet handed out again (the kernel's
+  // 4 * 64-entry stack, scaled to the model's lanes)
+  const bool recycle = early && !(C.variant & HVM_MODEL_NO_RECYCLE);
+  const size_t rec_cap = 4 * (size_t)L;
+  std::vector<u32> rec;
   for (u32 l = 0; l < L; ++l) {
     u32 i = w + l * W;
     if (i < nin && i < L * W) push_redex(lanes[l], bag_in[i], spills);
@@ -642,7 +667,7 @@ void Model::turn(u32 w) {
       // ---- P2: allocation ----
       if (need_n || need_v) {
         u32 oom = 0;
-        if (!wave_alloc(w, n_lo, v_lo, need_n, need_v, tbl_n, tbl_v, early, pre_n, pre_v, oom)) {
+        if (!wave_alloc(w, n_lo, v_lo, need_n, need_v, tbl_n, tbl_v, early, pre_n, pre_v, oom, rec)) {
           raise(oom);
           for (u32 l = 0; l < L; ++l) if (st[l].run) lanes[l].top++;
           break;
@@ -683,12 +708,16 @@ void Model::turn(u32 w) {
         x.run = false;
       }
     }
-    // peeked nodes are taken now
-    for (u32 l : order(L)) {
+    // peeked nodes are taken now: their slots go on the wave's recycling
+    // stack in lane order (zeroed once it is full), or are zeroed
+    for (u32 l = 0; l < L; ++l) {
       St& x = st[l];
-      if (x.run && peek(x)) {
-        if (x.take_a) node_store(hvm_get_val(x.a), 0);
-        if (x.take_b) node_store(hvm_get_val(x.b), 0);
+      if (!(x.run && peek(x))) continue;
+      for (u32 side = 0; side < 2; ++side) {
+        if (!(side ? x.take_b : x.take_a)) continue;
+        const u32 loc = hvm_get_val(side ? x.b : x.a);
+        if (recycle && loc < C.node_cap && rec.size() < rec_cap) { live[loc] = 0; rec.push_back(loc); }
+        else node_store(loc, 0);
       }
     }
     // ---- P4: creation ----
@@ -773,6 +802,9 @@ void Model::turn(u32 w) {
       }
     }
   }
+  // ---- end of turn: recycled slots not handed out are freed ----
+  for (u32 slot : rec) node_store(slot, 0);
+  rec.clear();
   // ---- end of turn: deques to the output bag, unpulled input carried ----
   for (auto& ln : lanes)
     for (u32 i = ln.bot; i != ln.top; ++i) {
@@ -896,6 +928,7 @@ extern "C" int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_
   m.node.assign((size_t)C.node_cap + 1, 0);
   m.vars.assign((size_t)C.vars_cap + 1, 0);
   m.node_gen.assign((size_t)C.node_cap + 1, 0);
+  m.live.assign((size_t)C.node_cap + 1, 0);
   m.bag[0].assign(C.bag_cap, 0);
   m.bag[1].assign(C.bag_cap, 0);
   m.ws.assign(C.waves, WaveState());

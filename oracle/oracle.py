@@ -153,6 +153,8 @@ MODEL_LATE_TAKES = 2
 MODEL_OVERLAP_SCAN = 4
 MODEL_TAKE_BEFORE_FIT = 8
 MODEL_XCHG_ONE = 16
+MODEL_NO_RECYCLE = 32
+MODEL_RECYCLE_KEEP = 64
 
 
 class _ModelCfg(C.Structure):

@@ -21,7 +21,7 @@ import pytest
 
 import programs as P
 
-ANOMALIES = ("zero_publish", "zero_enter", "take_zero", "putback", "clobber")
+ANOMALIES = ("zero_publish", "zero_enter", "take_zero", "putback", "clobber", "double_alloc")
 
 
 def mini_batch(seed, n, dmin=4, dmax=9, kmin=2, kmax=5):
@@ -79,7 +79,7 @@ def test_batch_idle_off_is_clean(batch, orc, seed):
     assert all(result_ok(t, e) for t, (_, e) in zip(r.results, nets))
 
 
-@pytest.mark.parametrize("variant", ["MODEL_XCHG_TWO", "MODEL_XCHG_ONE"])
+@pytest.mark.parametrize("variant", ["MODEL_XCHG_TWO", "MODEL_XCHG_ONE", "MODEL_NO_RECYCLE"])
 def test_batch_idle_off_exchange_takes_is_clean(batch, orc, variant):
     # the earlier take modes (exchanges, put back on a retry): of ANNI/COMM,
     # and of single-take rules and a CALL's copy (HVM_PEEK_TAKES=0)
@@ -87,6 +87,14 @@ def test_batch_idle_off_exchange_takes_is_clean(batch, orc, variant):
     r = orc.run_model(words, entries, seed=3, variant=getattr(orc, variant), **CFG)
     assert r.err == 0 and r.itrs == itrs
     assert r.events["take_zero"] == 0
+
+
+def test_recycled_slot_handed_out_twice_is_detected(batch, orc):
+    # a planted mistake: slots taken from the wave's recycling stack stay on
+    # it, so a later allocation hands them out again while their node lives
+    words, entries, nets, itrs = batch
+    r = orc.run_model(words, entries, seed=0, variant=orc.MODEL_RECYCLE_KEEP, **CFG)
+    assert r.events["double_alloc"] > 0
 
 
 def test_overlapping_scan_is_detected(batch, orc):
