@@ -93,6 +93,9 @@ typedef struct hvm_model_out {
                                   // on the wave's recycling stack, as HVM_NODE_RECYCLE does)
 #define HVM_MODEL_RECYCLE_KEEP 64u // a planted mistake: slots handed out from the recycling stack
                                   // stay on it (for the test that the double-allocation check works)
+#define HVM_MODEL_VRECYCLE_KEEP 128u // the same mistake on the var stack
+#define HVM_MODEL_VRECYCLE 256u  // var slots freed by links go on the wave's var stack (a build
+                                  // with HVM_VAR_RECYCLE > 0; off by default, as in the kernel)
 
 int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_t* entries, uint32_t n_nets,
                   const hvm_model_cfg* cfg, hvm_model_out* out);
@@ -156,6 +159,13 @@ struct Model {
   // a node slot holds a live node: set by a store of a node, cleared when the
   // node is taken (a zero store, an exchange, or a push on a recycling stack)
   std::vector<uint8_t> live;
+  // a var slot is handed out and not yet freed
+  std::vector<uint8_t> vlive;
+  // per wave: var slots freed by links and not yet handed out again (the
+  // kernel's 64-entry stack, scaled to the model's lanes)
+  std::vector<std::vector<u32>> vrec;
+  size_t vrec_cap = 0;
+  bool vrecycle = false;
   std::vector<u64> bag[2];
   // control block
   u32 n_in = 0, n_out = 0, in_head = 0, err = 0, parity = 0, max_steps = 0, idle_waves = 0, idle_quit = 0;
@@ -200,7 +210,27 @@ struct Model {
   }
   u32 vars_xchg(u32 val, u32 x) { yield(); u32 o = vars[vslot(val)]; vars[vslot(val)] = x; vtrace(1, vslot(val), o, x); return o; }
   u32 vars_load(u32 val) { yield(); return vars[vslot(val)]; }
-  void vars_store(u32 val, u32 x) { yield(); vtrace(2, vslot(val), vars[vslot(val)], x); vars[vslot(val)] = x; }
+  void vars_store(u32 val, u32 x) {
+    yield();
+    vtrace(2, vslot(val), vars[vslot(val)], x);
+    vars[vslot(val)] = x;
+    if (x == 0) vlive[vslot(val)] = 0;
+  }
+  void var_handed_out(u32 slot) {
+    if (vlive[slot]) { O->double_alloc++; event(6); }
+    vlive[slot] = 1;
+  }
+  // a var that a link owns is freed: onto the wave's stack, or zeroed
+  void var_release(u32 val) {
+    const u32 i = vslot(val);
+    std::vector<u32>& r = vrec[cur->w];
+    if (vrecycle && i >= C.root_vars && i < C.vars_cap && r.size() < vrec_cap) {
+      vlive[i] = 0;
+      r.push_back(i);
+      return;
+    }
+    vars_store(val, 0);
+  }
   template <typename T> bool cas(std::vector<T>& buf, u32 i, T expect, T v) {
     yield();
     if (buf[i] != expect) return false;
@@ -315,7 +345,7 @@ void Model::link_lanes(std::vector<std::vector<std::pair<u32, u32>>>& batches, s
           if (r == HVM_NONE || r == 0) {
             x.st[i] = PUBLISH;
           } else {
-            vars_store(hvm_get_val(x.pb[i]), 0);
+            var_release(hvm_get_val(x.pb[i]));
             x.pb[i] = r;
             x.st[i] = hvm_get_tag(r) == HVM_VAR ? (u32)ENTER : (u32)PUBLISH;
           }
@@ -324,7 +354,7 @@ void Model::link_lanes(std::vector<std::vector<std::pair<u32, u32>>>& batches, s
           if (r == HVM_NONE) {
             x.st[i] = DONE;
           } else {
-            vars_store(hvm_get_val(x.pa[i]), 0);
+            var_release(hvm_get_val(x.pa[i]));
             x.pa[i] = r;
             x.st[i] = ORIENT;
           }
@@ -364,8 +394,15 @@ bool Model::wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::v
     handed_out(tn[i]);
   }
   if (!(C.variant & HVM_MODEL_RECYCLE_KEEP)) rec.resize(rec.size() - got_n);
+  std::vector<u32>& vr = vrec[w];
+  const u32 got_v = std::min<u32>(need_v, (u32)vr.size());
+  for (u32 i = 0; i < got_v; ++i) {
+    tv[i] = vr[vr.size() - got_v + i];
+    var_handed_out(tv[i]);
+  }
+  if (!(C.variant & HVM_MODEL_VRECYCLE_KEEP)) vr.resize(vr.size() - got_v);
   Scan sn = {n_lo, node_slice, cs.node_cur, got_n, need_n, 0, false, cs.node_cur, cs.node_cur};
-  Scan sv = {v_lo, vars_slice, cs.vars_cur, 0, need_v, 0, false, cs.vars_cur, cs.vars_cur};
+  Scan sv = {v_lo, vars_slice, cs.vars_cur, got_v, need_v, 0, false, cs.vars_cur, cs.vars_cur};
   auto idx = [&](Scan& s, u32 lane, u32& rel) { rel = (s.cur - s.lo + lane) % s.len; return s.lo + rel; };
   auto window = [&](auto& buf, Scan& s, std::vector<u32>& tbl, u32 reserved, auto claim, auto* vals) {
     std::vector<u32> slot(L), rel(L);
@@ -385,7 +422,7 @@ bool Model::wave_alloc(u32 w, u32 n_lo, u32 v_lo, u32 need_n, u32 need_v, std::v
         tbl[s.got + r] = slot[l];
         if (s.pool) O->pool_claims++;
         if (&buf == (void*)&node) handed_out(slot[l]);
-        else vtrace(4, slot[l], (u32)vals[l], s.pool);
+        else { vtrace(4, slot[l], (u32)vals[l], s.pool); var_handed_out(slot[l]); }
         if (r == want - 1) s.cursor = s.lo + ((rel[l] + 1) % s.len);
       } else if (s.pool) {
         yield();
@@ -805,6 +842,8 @@ void Model::turn(u32 w) {
   // ---- end of turn: recycled slots not handed out are freed ----
   for (u32 slot : rec) node_store(slot, 0);
   rec.clear();
+  for (u32 slot : vrec[w]) vars_store(slot, 0);
+  vrec[w].clear();
   // ---- end of turn: deques to the output bag, unpulled input carried ----
   for (auto& ln : lanes)
     for (u32 i = ln.bot; i != ln.top; ++i) {
@@ -929,6 +968,10 @@ extern "C" int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_
   m.vars.assign((size_t)C.vars_cap + 1, 0);
   m.node_gen.assign((size_t)C.node_cap + 1, 0);
   m.live.assign((size_t)C.node_cap + 1, 0);
+  m.vlive.assign((size_t)C.vars_cap + 1, 0);
+  m.vrec.assign(C.waves, std::vector<u32>());
+  m.vrec_cap = (size_t)C.lanes;
+  m.vrecycle = (C.variant & HVM_MODEL_VRECYCLE) && !(C.variant & (HVM_MODEL_LATE_TAKES | HVM_MODEL_NO_RECYCLE));
   m.bag[0].assign(C.bag_cap, 0);
   m.bag[1].assign(C.bag_cap, 0);
   m.ws.assign(C.waves, WaveState());

@@ -155,6 +155,8 @@ MODEL_TAKE_BEFORE_FIT = 8
 MODEL_XCHG_ONE = 16
 MODEL_NO_RECYCLE = 32
 MODEL_RECYCLE_KEEP = 64
+MODEL_VRECYCLE_KEEP = 128
+MODEL_VRECYCLE = 256
 
 
 class _ModelCfg(C.Structure):

@@ -89,12 +89,25 @@ def test_batch_idle_off_exchange_takes_is_clean(batch, orc, variant):
     assert r.events["take_zero"] == 0
 
 
-def test_recycled_slot_handed_out_twice_is_detected(batch, orc):
-    # a planted mistake: slots taken from the wave's recycling stack stay on
-    # it, so a later allocation hands them out again while their node lives
+@pytest.mark.parametrize("variant", ["MODEL_RECYCLE_KEEP", "MODEL_VRECYCLE_KEEP"])
+def test_recycled_slot_handed_out_twice_is_detected(batch, orc, variant):
+    # a planted mistake: slots taken from the wave's node (var) recycling
+    # stack stay on it, so a later allocation hands them out again while they
+    # are in use
     words, entries, nets, itrs = batch
-    r = orc.run_model(words, entries, seed=0, variant=orc.MODEL_RECYCLE_KEEP, **CFG)
+    r = orc.run_model(words, entries, seed=0, variant=getattr(orc, variant) | orc.MODEL_VRECYCLE, **CFG)
     assert r.events["double_alloc"] > 0
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_batch_with_var_recycling_is_clean(batch, orc, seed):
+    # var slots freed by links recycled as well (a build with HVM_VAR_RECYCLE)
+    words, entries, nets, itrs = batch
+    r = orc.run_model(words, entries, seed=seed, variant=orc.MODEL_VRECYCLE, **CFG)
+    assert r.status == 0 and r.err == 0
+    assert {k: r.events[k] for k in ANOMALIES} == {k: 0 for k in ANOMALIES}
+    assert r.itrs == itrs
+    assert all(result_ok(t, e) for t, (_, e) in zip(r.results, nets))
 
 
 def test_overlapping_scan_is_detected(batch, orc):
