@@ -131,6 +131,31 @@ def atomic_roofline(name, itrs_per_s):
     return out
 
 
+def request_roofline(name, itrs_per_s):
+    """The evaluator against its memory-side request ceiling.  Per
+    interaction it issues E exchanges (TCC_EA0_ATOMIC), S other writes
+    (TCC_EA0_WRREQ, which counts the atomics too, minus E) and R reads
+    (TCC_EA0_RDREQ), from the committed counter summary; the device sustains
+    rE exchanges, rS device-scope stores and rR device-scope loads per second
+    on random slots of a 64 MiB buffer (hvm_hip_request_rate, 8-byte slots).
+    With those requests as the only cost an interaction takes E/rE + S/rS +
+    R/rR seconds of the memory side: the peak interaction rate."""
+    import hvm_hip
+    vals, src = profile_values(name)
+    e, w, r = (vals.get(k) for k in ("TCC_ATOMIC_per_interaction", "TCC_EA0_WRREQ_per_interaction",
+                                     "TCC_EA0_RDREQ_per_interaction"))
+    if not (e and w and r):
+        return None
+    slots = (64 << 20) // 8
+    rates = {k: hvm_hip.request_rate(slots, 8, op) for k, op in (("exchange", 0), ("store", 1), ("load", 2))}
+    mix = {"exchange": e, "store": w - e, "load": r}
+    peak = 1.0 / sum(mix[k] / rates[k] for k in mix)
+    return {"unit": "G interactions/s", "peak": round(peak / 1e9, 3), "achieved": round(itrs_per_s / 1e9, 3),
+            "frac": round(itrs_per_s / peak, 4),
+            "requests_per_interaction": {k: round(v, 3) for k, v in mix.items()},
+            "rates_measured_G_per_s": {k: round(v / 1e9, 2) for k, v in rates.items()}, "source": src}
+
+
 def golden_case(golden, gen, params):
     return next((c for c in golden["workloads"] if c["generator"] == gen and c["params"] == params), None)
 
@@ -294,7 +319,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "program": f"{gen}({', '.join(str(v) for v in params.values())})",
                    "itrs_per_step": itrs, "scheduler": args.sched, "mode": args.mode},
-        "roofline": {"bound": "hbm", "limiter": "latency of memory-side atomics (see atomic_roofline)",
+        "roofline": {"bound": "hbm", "limiter": "memory-side requests and their round trips (see request_roofline)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src, "kernel": kernel_name,
@@ -308,6 +333,7 @@ def main():
     }
     if args.mode == "interpreted":
         out["atomic_roofline"] = atomic_roofline(prof, itrs * args.steps / (kernel_ms * 1e-3))
+        out["request_roofline"] = request_roofline(prof, itrs * args.steps / (kernel_ms * 1e-3))
 
     if not args.only_headline:
         # ---- the 256-net batch, both schedulers, same protocol ----

@@ -254,6 +254,12 @@ int hvm_mlp_train(const float* params_in, const float* x, const uint32_t* action
  * flight.  The evaluator's takes and links are such exchanges. */
 int hvm_hip_atomic_rate(uint64_t slots, uint32_t bytes, double* ops_per_s);
 
+/* The request rooflines: the same measurement for op 0 (exchanges, as
+ * hvm_hip_atomic_rate), 1 (device-scope stores) or 2 (device-scope loads) —
+ * the three kinds of memory-side request the evaluator issues (links and
+ * pool claims; creations and frees; takes and the allocator's windows). */
+int hvm_hip_request_rate(uint64_t slots, uint32_t bytes, uint32_t op, double* ops_per_s);
+
 /* ------------------------------------------------------------------------ */
 /* Front end (the reference's Rust host, restated in C++)                   */
 /* ------------------------------------------------------------------------ */

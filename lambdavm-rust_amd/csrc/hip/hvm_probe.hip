@@ -1,8 +1,9 @@
-// hvm_probe.hip — measurement of the atomic roofline of the evaluator: the
-// rate at which the device performs device-scope exchanges on random slots of
-// a net-sized buffer, the operation the evaluator's takes and links consist
-// of.  Used by bench.py to report the evaluator's exchange rate against this
-// ceiling next to the HBM roofline.
+// hvm_probe.hip — measurement of the request rooflines of the evaluator: the
+// rates at which the device performs device-scope exchanges, stores and loads
+// on random slots of a net-sized buffer, the three kinds of memory-side
+// request the evaluator's takes, creations, frees and links consist of.  Used
+// by bench.py to report the evaluator's request rate against these ceilings
+// next to the HBM roofline.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -22,7 +23,8 @@ __device__ __forceinline__ uint32_t xorshift32(uint32_t x) {
 // kernel writes only if it hits an impossible pattern, so no exchange is dead.
 constexpr int HVM_PROBE_INFLIGHT = 8;
 
-template <typename T>
+// OP 0: exchanges, 1: stores, 2: loads (all device-scope, relaxed).
+template <typename T, int OP>
 __global__ __launch_bounds__(256) void atomic_probe_kernel(T* buf, uint32_t mask, uint32_t rounds, T* sink) {
   uint32_t x = (blockIdx.x * blockDim.x + threadIdx.x) * 2654435761u + 0x9E3779B9u;
   x = x ? x : 1u;
@@ -32,7 +34,9 @@ __global__ __launch_bounds__(256) void atomic_probe_kernel(T* buf, uint32_t mask
 #pragma unroll
     for (int k = 0; k < HVM_PROBE_INFLIGHT; ++k) {
       x = xorshift32(x);
-      v[k] = __hip_atomic_exchange(&buf[x & mask], (T)x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (OP == 0) v[k] = __hip_atomic_exchange(&buf[x & mask], (T)x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else if (OP == 1) { __hip_atomic_store(&buf[x & mask], (T)x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); v[k] = 0; }
+      else v[k] = __hip_atomic_load(&buf[x & mask], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int k = 0; k < HVM_PROBE_INFLIGHT; ++k) acc ^= v[k];
@@ -40,7 +44,7 @@ __global__ __launch_bounds__(256) void atomic_probe_kernel(T* buf, uint32_t mask
   if (acc == (T)0x5A5A5A5A3C3C3C3Cull) sink[0] = acc;
 }
 
-template <typename T>
+template <typename T, int OP>
 int run_probe(uint64_t slots, double* ops_per_s) {
   if (slots < 1024 || (slots & (slots - 1)) || slots > (1ull << 31)) return HVM_HIP_E_ARG;
   T* buf = nullptr;
@@ -53,9 +57,9 @@ int run_probe(uint64_t slots, double* ops_per_s) {
       hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
     const uint32_t mask = (uint32_t)(slots - 1);
     // a warm-up launch, then the timed one
-    hipLaunchKernelGGL(atomic_probe_kernel<T>, dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
+    hipLaunchKernelGGL((atomic_probe_kernel<T, OP>), dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
     bool good = hipGetLastError() == hipSuccess && hipEventRecord(e0, 0) == hipSuccess;
-    if (good) hipLaunchKernelGGL(atomic_probe_kernel<T>, dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
+    if (good) hipLaunchKernelGGL((atomic_probe_kernel<T, OP>), dim3(blocks), dim3(threads), 0, 0, buf, mask, rounds, sink);
     good = good && hipGetLastError() == hipSuccess && hipEventRecord(e1, 0) == hipSuccess;
     float ms = 0;
     if (good && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0) {
@@ -74,8 +78,15 @@ int run_probe(uint64_t slots, double* ops_per_s) {
 }  // namespace
 
 extern "C" int hvm_hip_atomic_rate(uint64_t slots, uint32_t bytes, double* ops_per_s) {
-  if (!ops_per_s) return HVM_HIP_E_ARG;
-  if (bytes == 4) return run_probe<uint32_t>(slots, ops_per_s);
-  if (bytes == 8) return run_probe<unsigned long long>(slots, ops_per_s);
+  return hvm_hip_request_rate(slots, bytes, 0, ops_per_s);
+}
+
+extern "C" int hvm_hip_request_rate(uint64_t slots, uint32_t bytes, uint32_t op, double* ops_per_s) {
+  if (!ops_per_s || op > 2) return HVM_HIP_E_ARG;
+  if (bytes == 4) return op == 0 ? run_probe<uint32_t, 0>(slots, ops_per_s)
+                       : op == 1 ? run_probe<uint32_t, 1>(slots, ops_per_s) : run_probe<uint32_t, 2>(slots, ops_per_s);
+  if (bytes == 8) return op == 0 ? run_probe<unsigned long long, 0>(slots, ops_per_s)
+                       : op == 1 ? run_probe<unsigned long long, 1>(slots, ops_per_s)
+                                 : run_probe<unsigned long long, 2>(slots, ops_per_s);
   return HVM_HIP_E_ARG;
 }

@@ -356,6 +356,18 @@ def atomic_rate(slots: int, nbytes: int) -> float:
     return out.value
 
 
+def request_rate(slots: int, nbytes: int, op: int) -> float:
+    """Device-scope requests per second on random slots (hvm_hip_request_rate):
+    op 0 exchanges, 1 stores, 2 loads."""
+    L = lib()
+    L.hvm_hip_request_rate.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]
+    out = C.c_double(0)
+    rc = L.hvm_hip_request_rate(slots, nbytes, op, C.byref(out))
+    if rc != 0:
+        raise HvmError(f"hvm_hip_request_rate failed: {STATUS.get(rc, rc)}")
+    return out.value
+
+
 def mlp_forward(params_fp16, x_rows):
     """x_rows: list of 32-float rows → list of 4-float predictions."""
     n = len(x_rows)
