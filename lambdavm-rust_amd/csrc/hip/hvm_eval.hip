@@ -178,6 +178,12 @@ __device__ __forceinline__ u32 var_xchg(const HvmTurnArgs& A, LocalNet* N, u32 v
   if (LOC && is_loc_val(val)) return __hip_atomic_exchange(&N->vars[loc_v(A, val)], x, __ATOMIC_RELAXED, WGS);
   return vars_xchg(A, val, x);
 }
+// A coherent read of a var (the link loop's ENTER step; see HVM_ENTER_LOAD).
+template <bool LOC>
+__device__ __forceinline__ u32 var_load(const HvmTurnArgs& A, LocalNet* N, u32 val) {
+  if (LOC && is_loc_val(val)) return __hip_atomic_load(&N->vars[loc_v(A, val)], __ATOMIC_RELAXED, WGS);
+  return __hip_atomic_load(&A.vars[vars_slot(A, val)], __ATOMIC_RELAXED, AGENT);
+}
 template <bool LOC>
 __device__ __forceinline__ void var_put(const HvmTurnArgs& A, LocalNet* N, u32 val, u32 x) {
   if (LOC && is_loc_val(val)) { __hip_atomic_store(&N->vars[loc_v(A, val)], x, __ATOMIC_RELAXED, WGS); return; }
@@ -294,6 +300,19 @@ enum { LK_DONE = 0, LK_ENTER = 1, LK_PUBLISH = 2, LK_ORIENT = 3 };
 #ifndef HVM_LINK_ENTER
 #define HVM_LINK_ENTER 1
 #endif
+// ENTER reads B with a load instead of exchanging NONE into it.  An unbound B
+// reads NONE either way (the exchange writes NONE over NONE).  A bound B holds
+// the port its other end published, and that end is done with it, so only
+// this link touches B again: it takes the port and frees B with a store, as
+// after the exchange.  A load that still sees NONE after the other end's
+// publish only makes A point to B, a chain a later ENTER follows.  It trades a
+// memory-side exchange for a load: measured (median of 5 evaluations, two
+// alternating rounds) 4-7% slower on every workload (LSD 2^18 205 -> 216 ms):
+// a device-scope load's round trip is longer than an exchange's, and the
+// link loop waits for it.  Off by default.
+#ifndef HVM_ENTER_LOAD
+#define HVM_ENTER_LOAD 0
+#endif
 
 // With the local net, a pair about to publish an HBM var A with a local B
 // (src/hvm.cu:1061-1072) either swaps (B is a var: the local var points to
@@ -341,7 +360,8 @@ __device__ __forceinline__ void link_batch(WaveLds& L, LocalNet* N, Lane& ln, co
       const u32 v = hvm_get_val(ent ? pb[i] : pa[i]);
       const u32 x = ent ? (u32)HVM_NONE : pb[i];
       r[i] = 0;
-      if (ent || st[i] == LK_PUBLISH) r[i] = var_xchg<LOC>(A, N, v, x);
+      if (HVM_ENTER_LOAD && ent) r[i] = var_load<LOC>(A, N, v);
+      else if (ent || st[i] == LK_PUBLISH) r[i] = var_xchg<LOC>(A, N, v, x);
     }
     busy = false;
 #pragma unroll

@@ -94,6 +94,8 @@ typedef struct hvm_model_out {
 #define HVM_MODEL_RECYCLE_KEEP 64u // a planted mistake: slots handed out from the recycling stack
                                   // stay on it (for the test that the double-allocation check works)
 #define HVM_MODEL_VRECYCLE_KEEP 128u // the same mistake on the var stack
+#define HVM_MODEL_ENTER_LOAD 512u // the link loop's ENTER loads B (a build with HVM_ENTER_LOAD;
+                                  // else it exchanges NONE into B, as by default)
 #define HVM_MODEL_VRECYCLE 256u  // var slots freed by links go on the wave's var stack (a build
                                   // with HVM_VAR_RECYCLE > 0; off by default, as in the kernel)
 
@@ -330,7 +332,9 @@ void Model::link_lanes(std::vector<std::vector<std::pair<u32, u32>>>& batches, s
     for (auto& op : ops) {
       B& x = b[op.first];
       u32 i = op.second;
-      if (x.st[i] == ENTER) res[op.first * 4 + i] = vars_xchg(hvm_get_val(x.pb[i]), HVM_NONE);
+      if (x.st[i] == ENTER)
+        res[op.first * 4 + i] = (C.variant & HVM_MODEL_ENTER_LOAD) ? vars_load(hvm_get_val(x.pb[i]))
+                                                                   : vars_xchg(hvm_get_val(x.pb[i]), HVM_NONE);
       else res[op.first * 4 + i] = vars_xchg(hvm_get_val(x.pa[i]), x.pb[i]);
     }
     // results, frees

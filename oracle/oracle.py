@@ -157,6 +157,7 @@ MODEL_NO_RECYCLE = 32
 MODEL_RECYCLE_KEEP = 64
 MODEL_VRECYCLE_KEEP = 128
 MODEL_VRECYCLE = 256
+MODEL_ENTER_LOAD = 512
 
 
 class _ModelCfg(C.Structure):

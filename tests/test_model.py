@@ -79,10 +79,12 @@ def test_batch_idle_off_is_clean(batch, orc, seed):
     assert all(result_ok(t, e) for t, (_, e) in zip(r.results, nets))
 
 
-@pytest.mark.parametrize("variant", ["MODEL_XCHG_TWO", "MODEL_XCHG_ONE", "MODEL_NO_RECYCLE"])
+@pytest.mark.parametrize("variant", ["MODEL_XCHG_TWO", "MODEL_XCHG_ONE", "MODEL_NO_RECYCLE", "MODEL_ENTER_LOAD"])
 def test_batch_idle_off_exchange_takes_is_clean(batch, orc, variant):
     # the earlier take modes (exchanges, put back on a retry): of ANNI/COMM,
-    # and of single-take rules and a CALL's copy (HVM_PEEK_TAKES=0)
+    # and of single-take rules and a CALL's copy (HVM_PEEK_TAKES=0); node
+    # slots zeroed at commit (HVM_NODE_RECYCLE=0); ENTER by load
+    # (HVM_ENTER_LOAD=1)
     words, entries, nets, itrs = batch
     r = orc.run_model(words, entries, seed=3, variant=getattr(orc, variant), **CFG)
     assert r.err == 0 and r.itrs == itrs
