@@ -290,7 +290,9 @@ __device__ __forceinline__ u32 enter(const HvmTurnArgs& A, LocalNet* N, u32 var)
 // one new redex; the redexes are pushed in pair order once every pair is done,
 // so the deque receives them in the order a one-at-a-time linker would push
 // them (the deque is LIFO, and that order keeps the evaluation depth-first).
+#ifndef HVM_LB
 #define HVM_LB 4
+#endif
 enum { LK_DONE = 0, LK_ENTER = 1, LK_PUBLISH = 2, LK_ORIENT = 3 };
 // Before publishing A ~> B the reference first follows B while it is a bound
 // var (src/hvm.c:796-808).  With 0 a link publishes at once, one exchange
@@ -1194,7 +1196,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
           } else {
             // the def's redexes, then its root against the caller's port,
             // HVM_LB links at a time (the first ones were loaded ahead)
-            static_assert(HVM_PRE_RBAG == HVM_LB, "one batch of redexes is loaded ahead");
+            static_assert(HVM_LB >= HVM_PRE_RBAG, "the redexes loaded ahead fit one batch");
 #pragma unroll
             for (int k = 0; k < HVM_PRE_RBAG; ++k) {
               if ((u32)k < d.rlen) {
