@@ -260,6 +260,10 @@ int hvm_hip_atomic_rate(uint64_t slots, uint32_t bytes, double* ops_per_s);
  * pool claims; creations and frees; takes and the allocator's windows). */
 int hvm_hip_request_rate(uint64_t slots, uint32_t bytes, uint32_t op, double* ops_per_s);
 
+/* The attainable HBM bandwidth: bytes read plus bytes written per second by
+ * the best of 5 device-to-device copies of `bytes` (1 MiB..8 GiB). */
+int hvm_hip_copy_rate(uint64_t bytes, double* bytes_per_s);
+
 /* ------------------------------------------------------------------------ */
 /* Front end (the reference's Rust host, restated in C++)                   */
 /* ------------------------------------------------------------------------ */

@@ -73,6 +73,19 @@ struct HvmDevDefX {
   unsigned long long rbag[HVM_PRE_RBAG];  // rbag_buf[0, 4), zero past rlen
 };
 
+// The evaluator's counters (HvmDevStats, HvmTurnStats) are kept in
+// HVM_STAT_STRIPES copies: wave w adds to copy w % HVM_STAT_STRIPES, and the
+// readers sum the copies (the host the totals, the between kernel the turn's
+// counters).  With one copy the 2048 waves of a turn ended with ~34 atomics
+// each on the same ~34 addresses, which the memory side serialises (~29 µs
+// per turn, measured: church_loop(12, 12), 1,941 turns, 426 ms; 370 ms
+// without the atomics, 395 ms with 64 copies).  256 copies cost the between
+// kernel more than they save (421 ms), and so did keeping the totals out of
+// the turn kernel, added once per turn by the between kernel (402 ms).
+#ifndef HVM_STAT_STRIPES
+#define HVM_STAT_STRIPES 64
+#endif
+
 // Statistic slots accumulated by the evaluator (same order as the oracle).
 enum {
   HVM_K_CALL = 0, HVM_K_CALL_COPY, HVM_K_VOID, HVM_K_ERAS, HVM_K_ANNI, HVM_K_COMM,
@@ -177,7 +190,7 @@ struct HvmSched {
   float* trace;                  // trace_cap * HVM_TRACE_REC floats, or null
   float qavg[HVM_ACTIONS];       // learned mode: the network's outputs averaged over the turns
   uint32_t qn;                   // of this normalisation (exponentially), and their count
-  HvmTurnStats turn;
+  HvmTurnStats turn[HVM_STAT_STRIPES];   // summed by the between kernel
 };
 
 struct HvmTurnArgs {

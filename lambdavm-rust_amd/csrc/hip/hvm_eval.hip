@@ -1422,25 +1422,26 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     retries = wave_sum64(retries);
     u64 s_leaks = wc.leaks, s_saves = wc.saves;
     if (lane == 0) {
-      if (s_itrs) atomicAdd(&A.stats->itrs, s_itrs);
-      if (s_bytes) atomicAdd(&A.stats->bytes, s_bytes);
-      if (s_spill) atomicAdd(&A.stats->spills, s_spill);
+      HvmDevStats* St = A.stats + (w & (HVM_STAT_STRIPES - 1));
+      if (s_itrs) atomicAdd(&St->itrs, s_itrs);
+      if (s_bytes) atomicAdd(&St->bytes, s_bytes);
+      if (s_spill) atomicAdd(&St->spills, s_spill);
 #pragma unroll
-      for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&A.stats->kind[k], kinds[k]);
-      if (steps) atomicAdd(&A.stats->steps, steps);
-      if (defers) atomicAdd(&A.stats->defers, defers);
-      if (retries) atomicAdd(&A.stats->retries, retries);
-      if (probes) atomicAdd(&A.stats->probes, probes);
-      if (s_leaks) atomicAdd(&A.stats->leaks, s_leaks);
-      if (s_saves) atomicAdd(&A.stats->saves, s_saves);
-      if (left) atomicAdd(&A.stats->local_left, left);
+      for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&St->kind[k], kinds[k]);
+      if (steps) atomicAdd(&St->steps, steps);
+      if (defers) atomicAdd(&St->defers, defers);
+      if (retries) atomicAdd(&St->retries, retries);
+      if (probes) atomicAdd(&St->probes, probes);
+      if (s_leaks) atomicAdd(&St->leaks, s_leaks);
+      if (s_saves) atomicAdd(&St->saves, s_saves);
+      if (left) atomicAdd(&St->local_left, left);
       if (prof) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) if (ph[k]) atomicAdd(&A.stats->phase[k], ph[k]);
+        for (int k = 0; k < 6; ++k) if (ph[k]) atomicAdd(&St->phase[k], ph[k]);
       }
       A.wave_state[w] = ws;
       // per-turn counters for the scheduler
-      HvmTurnStats* T = &A.sched->turn;
+      HvmTurnStats* T = &A.sched->turn[w & (HVM_STAT_STRIPES - 1)];
       if (s_itrs) atomicAdd(&T->itrs, s_itrs);
       if (steps) atomicAdd(&T->wave_steps, steps);
       if (lane_steps) atomicAdd(&T->lane_steps, lane_steps);
@@ -1574,9 +1575,8 @@ __host__ __device__ __forceinline__ u32 plain_action(u32 n, u32 nwaves) {
 __device__ __forceinline__ float log2p(float x) { return __log2f(1.0f + x); }
 
 // Scheduler features of the turn that just ended (all in [0, ~1]).
-__device__ void sched_features(const HvmCtl* c, const HvmSched* s, u32 n_next, u32 nwaves, float dt_us,
-                               float* f) {
-  const HvmTurnStats& T = s->turn;
+__device__ void sched_features(const HvmCtl* c, const HvmSched* s, const HvmTurnStats& T, u32 n_next, u32 nwaves,
+                               float dt_us, float* f) {
   float lanes = (float)nwaves * HVM_WAVE;
   float itrs = (float)T.itrs;
   float ws = (float)(T.wave_steps ? T.wave_steps : 1);
@@ -1611,9 +1611,28 @@ __device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__u
 __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u32 clock_khz) {
   __shared__ float f[HVM_FEAT], h1[HVM_HID], h2[HVM_HID], y[HVM_ACTIONS];
   __shared__ u32 s_go, s_n, s_steady;
+  __shared__ HvmTurnStats tsum;
   HvmCtl* c = A.ctl;
   HvmSched* s = A.sched;
   const u32 t = threadIdx.x;
+  // the turn's counters: lane t sums copies t, t + 64, ... into tsum, and
+  // clears them
+  static_assert(HVM_STAT_STRIPES % HVM_WAVE == 0, "copies of the turn counters: a multiple of the lanes");
+  static_assert(sizeof(HvmTurnStats) % 8 == 0, "turn counters are u64");
+  {
+    unsigned long long* sum = (unsigned long long*)&tsum;
+    for (u32 k = 0; k < sizeof(HvmTurnStats) / 8; ++k) {
+      unsigned long long v = 0;
+      for (u32 j = t; j < HVM_STAT_STRIPES; j += HVM_WAVE) {
+        unsigned long long* mine = (unsigned long long*)&s->turn[j];
+        v += mine[k];
+        mine[k] = 0;
+      }
+      v = wave_sum64(v);
+      if (t == 0) sum[k] = v;
+    }
+  }
+  __syncthreads();
   if (t == 0) {
     s_go = 0;
     s_steady = 0;
@@ -1634,11 +1653,11 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
         float dt_us = s->last_clock ? (float)(now - s->last_clock) * 1000.0f / (float)clock_khz : 0.0f;
         if (s->trace && s->last_clock && s->trace_len > 0 && s->trace_len <= s->trace_cap) {
           float* rec = s->trace + (size_t)(s->trace_len - 1) * HVM_TRACE_REC;
-          rec[HVM_FEAT + 1] = __log2f((1.0f + (float)s->turn.itrs) / fmaxf(dt_us, 1.0f));
-          rec[HVM_FEAT + 2] = (float)s->turn.itrs;
+          rec[HVM_FEAT + 1] = __log2f((1.0f + (float)tsum.itrs) / fmaxf(dt_us, 1.0f));
+          rec[HVM_FEAT + 2] = (float)tsum.itrs;
           rec[HVM_FEAT + 3] = dt_us;
         }
-        sched_features(c, s, n, A.nwaves, dt_us, f);
+        sched_features(c, s, tsum, n, A.nwaves, dt_us, f);
         // few redexes, not growing: independent sequential chains, which a
         // redistribution cannot spread; the next turn runs without idle quit
         s_steady = (u64)n * HVM_STEADY_DEN <= (u64)A.nwaves * HVM_STEADY_NUM &&
@@ -1653,8 +1672,6 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
         else s_go = 1;
         s->last_clock = now;
       }
-      HvmTurnStats z = {};
-      s->turn = z;
     }
   }
   __syncthreads();

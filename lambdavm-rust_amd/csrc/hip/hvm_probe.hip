@@ -77,6 +77,38 @@ int run_probe(uint64_t slots, double* ops_per_s) {
 
 }  // namespace
 
+extern "C" int hvm_hip_copy_rate(uint64_t bytes, double* bytes_per_s) {
+  if (!bytes_per_s || bytes < (1u << 20) || bytes > (8ull << 30)) return HVM_HIP_E_ARG;
+  void* src = nullptr;
+  void* dst = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = HVM_HIP_E_DEVICE;
+  if (hipMalloc(&src, bytes) == hipSuccess && hipMalloc(&dst, bytes) == hipSuccess &&
+      hipMemset(src, 1, bytes) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+      hipEventCreate(&e1) == hipSuccess && hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess) {
+    // best of 5 device-to-device copies: bytes read plus bytes written
+    float best = 0;
+    bool good = true;
+    for (int i = 0; i < 5 && good; ++i) {
+      float ms = 0;
+      good = hipEventRecord(e0, 0) == hipSuccess &&
+             hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, 0) == hipSuccess &&
+             hipEventRecord(e1, 0) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0;
+      if (good && (best == 0 || ms < best)) best = ms;
+    }
+    if (good) {
+      *bytes_per_s = 2.0 * (double)bytes / (best * 1e-3);
+      rc = HVM_HIP_OK;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return rc;
+}
+
 extern "C" int hvm_hip_atomic_rate(uint64_t slots, uint32_t bytes, double* ops_per_s) {
   return hvm_hip_request_rate(slots, bytes, 0, ops_per_s);
 }

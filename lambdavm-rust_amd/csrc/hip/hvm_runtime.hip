@@ -203,7 +203,7 @@ struct hvm_hip_ctx {
       bag[1] = (u64*)carve((size_t)bag_cap * 8);
       ctl = (HvmCtl*)carve(sizeof(HvmCtl));
       wave_state = (HvmWaveState*)carve(nwaves * sizeof(HvmWaveState));
-      stats = (HvmDevStats*)carve(sizeof(HvmDevStats));
+      stats = (HvmDevStats*)carve(sizeof(HvmDevStats) * HVM_STAT_STRIPES);
       sched = (HvmSched*)carve(sizeof(HvmSched));
       d_params = (unsigned short*)carve(HVM_MLP_PARAMS * sizeof(unsigned short));
       s_book = (unsigned char*)carve(kStaticBook);
@@ -226,7 +226,7 @@ struct hvm_hip_ctx {
     if (!ck(hipMalloc(&ctl, sizeof(HvmCtl)), "hipMalloc(ctl)")) return false;
     if (!ck(hipMalloc(&wave_state, nwaves * sizeof(HvmWaveState)), "hipMalloc(wave_state)")) return false;
     if (local && !ck(hipMalloc(&lspill, (size_t)nwaves * HVM_LSPILL * 8), "hipMalloc(lspill)")) return false;
-    if (!ck(hipMalloc(&stats, sizeof(HvmDevStats)), "hipMalloc(stats)")) return false;
+    if (!ck(hipMalloc(&stats, sizeof(HvmDevStats) * HVM_STAT_STRIPES), "hipMalloc(stats)")) return false;
     if (!ck(hipMalloc(&sched, sizeof(HvmSched)), "hipMalloc(sched)")) return false;
     if (!ck(hipMalloc(&d_params, HVM_MLP_PARAMS * sizeof(unsigned short)), "hipMalloc(params)")) return false;
     return true;
@@ -374,9 +374,25 @@ struct hvm_hip_ctx {
     return ck(hipStreamSynchronize(stream), "reset sync");
   }
 
+  // The evaluator's counters, summed over their copies (HVM_STAT_STRIPES).
+  bool read_stats(HvmDevStats& s) {
+    static_assert(sizeof(HvmDevStats) % 8 == 0, "counters are u64");
+    std::vector<HvmDevStats> all(HVM_STAT_STRIPES);
+    memset(&s, 0, sizeof s);
+    if (!ck(hipMemcpy(all.data(), stats, sizeof(HvmDevStats) * HVM_STAT_STRIPES, hipMemcpyDeviceToHost),
+            "stats copy"))
+      return false;
+    unsigned long long* sum = (unsigned long long*)&s;
+    for (const HvmDevStats& x : all) {
+      const unsigned long long* v = (const unsigned long long*)&x;
+      for (size_t k = 0; k < sizeof(HvmDevStats) / 8; ++k) sum[k] += v[k];
+    }
+    return true;
+  }
+
   bool reset_net() {
     copy_failed = false;
-    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return false;
+    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats) * HVM_STAT_STRIPES, stream), "clear stats")) return false;
     return reset_heap();
   }
 
@@ -1063,8 +1079,7 @@ struct hvm_hip_ctx {
     int rc = run_io();
     double t_eval = now_s();
     HvmDevStats s;
-    memset(&s, 0, sizeof s);
-    if (!ck(hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost), "stats copy") && rc == 0) rc = HVM_HIP_E_DEVICE;
+    if (!read_stats(s) && rc == 0) rc = HVM_HIP_E_DEVICE;
     if (out) {
       memset(out, 0, sizeof *out);
       out->itrs = s.itrs;
@@ -1129,8 +1144,7 @@ struct hvm_hip_ctx {
 
   bool fill_out(hvm_hip_out* out, double t0, double t_eval) {
     HvmDevStats s;
-    memset(&s, 0, sizeof s);
-    const bool good = ck(hipMemcpy(&s, stats, sizeof s, hipMemcpyDeviceToHost), "stats copy");
+    const bool good = read_stats(s);
     memset(out, 0, sizeof *out);
     out->itrs = s.itrs;
     for (int k = 0; k < HVM_HIP_KINDS; ++k) out->kind[k] = s.kind[k];
@@ -1163,7 +1177,7 @@ struct hvm_hip_ctx {
     launches = 0;
     normalizes = 0;
     eval_started = false;
-    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats), stream), "clear stats")) return HVM_HIP_E_DEVICE;
+    if (!ck(hipMemsetAsync(stats, 0, sizeof(HvmDevStats) * HVM_STAT_STRIPES, stream), "clear stats")) return HVM_HIP_E_DEVICE;
     double t0 = now_s();
     deadline = cfg.time_limit_s ? t0 + cfg.time_limit_s : 0;
     // admission budget: a quarter of the node capacity, leaving headroom for

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstring>
 #include <unordered_map>
+#include <vector>
 
 #include "../../../include/hvm_hip.h"
 #include "../front/hvm_front.hpp"
@@ -494,14 +495,15 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
   const u64 s_bytes = wave_sum64(ln.bytes), s_spill = wave_sum64(ln.spills);
   defers = wave_sum64(defers);
   if (lane == 0) {
-    if (s_itrs) atomicAdd(&A.stats->itrs, s_itrs);
-    if (s_bytes) atomicAdd(&A.stats->bytes, s_bytes);
-    if (s_spill) atomicAdd(&A.stats->spills, s_spill);
+    HvmDevStats* St = A.stats + (w & (HVM_STAT_STRIPES - 1));   // one of the counters' copies
+    if (s_itrs) atomicAdd(&St->itrs, s_itrs);
+    if (s_bytes) atomicAdd(&St->bytes, s_bytes);
+    if (s_spill) atomicAdd(&St->spills, s_spill);
 #pragma unroll
-    for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&A.stats->kind[k], kinds[k]);
-    if (steps) atomicAdd(&A.stats->steps, steps);
-    if (defers) atomicAdd(&A.stats->defers, defers);
-    if (retries) atomicAdd(&A.stats->retries, retries);
+    for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&St->kind[k], kinds[k]);
+    if (steps) atomicAdd(&St->steps, steps);
+    if (defers) atomicAdd(&St->defers, defers);
+    if (retries) atomicAdd(&St->retries, retries);
     A.wave_state[w] = ws;
   }
 }
@@ -575,7 +577,7 @@ bool HvmWide::create(uint64_t node_cap, uint64_t vars_cap, uint32_t bag_cap, uin
          ok(hipMalloc(&a.bag[1], (size_t)bag_cap * 16), "hipMalloc(wide bag)") &&
          ok(hipMalloc(&a.ctl, sizeof(HvmWideCtl)), "hipMalloc(wide ctl)") &&
          ok(hipMalloc(&a.wave_state, nw * sizeof(HvmWaveState)), "hipMalloc(wide wave state)") &&
-         ok(hipMalloc(&a.stats, sizeof(HvmDevStats)), "hipMalloc(wide stats)");
+         ok(hipMalloc(&a.stats, sizeof(HvmDevStats) * HVM_STAT_STRIPES), "hipMalloc(wide stats)");
 }
 
 void HvmWide::destroy() {
@@ -684,7 +686,7 @@ int HvmWide::normalize(const HvmDevDefX* defs, const unsigned long long* pairs, 
   if (!ok(hipMemsetAsync(a.node, 0, ((size_t)a.node_cap + 1) * 16, stream), "clear wide net") ||
       !ok(hipMemsetAsync(a.vars, 0, ((size_t)a.vars_cap + 1) * 8, stream), "clear wide net") ||
       !ok(hipMemsetAsync(a.wave_state, 0, a.nwaves * sizeof(HvmWaveState), stream), "clear wide net") ||
-      !ok(hipMemsetAsync(a.stats, 0, sizeof(HvmDevStats), stream), "clear wide stats"))
+      !ok(hipMemsetAsync(a.stats, 0, sizeof(HvmDevStats) * HVM_STAT_STRIPES, stream), "clear wide stats"))
     return HVM_HIP_E_DEVICE;
   if (!boot(wport(HVM_REF, 0), W_ROOT)) return HVM_HIP_E_DEVICE;
   int rc = run();
@@ -727,7 +729,19 @@ int HvmWide::normalize(const HvmDevDefX* defs, const unsigned long long* pairs, 
   }
   n_turns = turns;
   n_launches = launches;
-  if (!ok(hipMemcpy(&out, a.stats, sizeof out, hipMemcpyDeviceToHost), "wide stats")) return HVM_HIP_E_DEVICE;
+  {
+    // the counters' copies, summed
+    std::vector<HvmDevStats> all(HVM_STAT_STRIPES);
+    if (!ok(hipMemcpy(all.data(), a.stats, sizeof(HvmDevStats) * HVM_STAT_STRIPES, hipMemcpyDeviceToHost),
+            "wide stats"))
+      return HVM_HIP_E_DEVICE;
+    memset(&out, 0, sizeof out);
+    unsigned long long* sum = (unsigned long long*)&out;
+    for (const HvmDevStats& x : all) {
+      const unsigned long long* v = (const unsigned long long*)&x;
+      for (size_t k = 0; k < sizeof(HvmDevStats) / 8; ++k) sum[k] += v[k];
+    }
+  }
   return rc;
 }
 

@@ -356,6 +356,18 @@ def atomic_rate(slots: int, nbytes: int) -> float:
     return out.value
 
 
+def copy_rate(nbytes: int) -> float:
+    """Bytes read plus written per second by a device-to-device copy of
+    `nbytes` (hvm_hip_copy_rate, best of 5)."""
+    L = lib()
+    L.hvm_hip_copy_rate.argtypes = [C.c_uint64, C.POINTER(C.c_double)]
+    out = C.c_double(0)
+    rc = L.hvm_hip_copy_rate(nbytes, C.byref(out))
+    if rc != 0:
+        raise HvmError(f"hvm_hip_copy_rate failed: {STATUS.get(rc, rc)}")
+    return out.value
+
+
 def request_rate(slots: int, nbytes: int, op: int) -> float:
     """Device-scope requests per second on random slots (hvm_hip_request_rate):
     op 0 exchanges, 1 stores, 2 loads."""
