@@ -1620,16 +1620,26 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
   static_assert(HVM_STAT_STRIPES % HVM_WAVE == 0, "copies of the turn counters: a multiple of the lanes");
   static_assert(sizeof(HvmTurnStats) % 8 == 0, "turn counters are u64");
   {
+    // every load in flight at once, then the clears, then the wave sums
+    constexpr u32 NF = sizeof(HvmTurnStats) / 8;
+    unsigned long long v[NF];
+#pragma unroll
+    for (u32 k = 0; k < NF; ++k) v[k] = 0;
+    for (u32 j = t; j < HVM_STAT_STRIPES; j += HVM_WAVE) {
+      const unsigned long long* mine = (const unsigned long long*)&s->turn[j];
+#pragma unroll
+      for (u32 k = 0; k < NF; ++k) v[k] += mine[k];
+    }
+    for (u32 j = t; j < HVM_STAT_STRIPES; j += HVM_WAVE) {
+      unsigned long long* mine = (unsigned long long*)&s->turn[j];
+#pragma unroll
+      for (u32 k = 0; k < NF; ++k) mine[k] = 0;
+    }
     unsigned long long* sum = (unsigned long long*)&tsum;
-    for (u32 k = 0; k < sizeof(HvmTurnStats) / 8; ++k) {
-      unsigned long long v = 0;
-      for (u32 j = t; j < HVM_STAT_STRIPES; j += HVM_WAVE) {
-        unsigned long long* mine = (unsigned long long*)&s->turn[j];
-        v += mine[k];
-        mine[k] = 0;
-      }
-      v = wave_sum64(v);
-      if (t == 0) sum[k] = v;
+#pragma unroll
+    for (u32 k = 0; k < NF; ++k) {
+      const unsigned long long x = wave_sum64(v[k]);
+      if (t == 0) sum[k] = x;
     }
   }
   __syncthreads();
