@@ -73,15 +73,16 @@ struct HvmDevDefX {
   unsigned long long rbag[HVM_PRE_RBAG];  // rbag_buf[0, 4), zero past rlen
 };
 
-// The evaluator's counters (HvmDevStats, HvmTurnStats) are kept in
-// HVM_STAT_STRIPES copies: wave w adds to copy w % HVM_STAT_STRIPES, and the
-// readers sum the copies (the host the totals, the between kernel the turn's
-// counters).  With one copy the 2048 waves of a turn ended with ~34 atomics
-// each on the same ~34 addresses, which the memory side serialises (~29 µs
-// per turn, measured: church_loop(12, 12), 1,941 turns, 426 ms; 370 ms
-// without the atomics, 395 ms with 64 copies).  256 copies cost the between
-// kernel more than they save (421 ms), and so did keeping the totals out of
-// the turn kernel, added once per turn by the between kernel (402 ms).
+// The evaluator's counters (HvmDevStats) are kept in HVM_STAT_STRIPES
+// copies: wave w adds its counters to copy w % HVM_STAT_STRIPES at the end of
+// its turn, and the readers sum the copies (the host for the totals; the
+// between kernel every turn, for the scheduler, which takes the difference
+// from the previous turn's sums).  With one copy, and a second set of
+// per-turn counters, the 2048 waves of a turn ended with ~34 atomics each on
+// the same ~34 addresses, which the memory side serialises (measured on
+// church_loop(12, 12), 1,941 turns: 426 ms; 370 ms without the atomics,
+// 395 ms with 64 copies).  256 copies cost the between kernel more than they
+// save (421 ms).
 #ifndef HVM_STAT_STRIPES
 #define HVM_STAT_STRIPES 64
 #endif
@@ -108,6 +109,10 @@ struct HvmDevStats {
   unsigned long long leaks;       // LEAK interactions (local node copied to HBM), not in itrs
   unsigned long long saves;       // redexes moved out of a local net at the end of a turn
   unsigned long long local_left;  // local slots still in use after a wave's evacuation
+  // for the scheduler's features
+  unsigned long long lane_steps;   // lanes with work, summed over the wave-steps
+  unsigned long long wave_left[4]; // waves that ended a turn leaving 0, 1-64, 65-128, >128 redexes
+  unsigned long long alloc_slots;  // slots the allocator handed out
 };
 
 // Error flags (bitwise) reported by the evaluator.
@@ -161,8 +166,8 @@ struct HvmCtl {
 
 enum { HVM_SCHED_PLAIN = 0, HVM_SCHED_LEARNED = 1, HVM_SCHED_EXPLORE = 2, HVM_SCHED_FIXED = 3 };
 
-// Per-turn counters, summed by the turn kernel and consumed (then cleared)
-// by the between kernel.
+// Per-turn counters (the scheduler's view of a turn): the difference of the
+// evaluator's summed counters between two decisions.
 struct HvmTurnStats {
   unsigned long long itrs;
   unsigned long long wave_steps;   // steps in which a wave had work
@@ -190,8 +195,21 @@ struct HvmSched {
   float* trace;                  // trace_cap * HVM_TRACE_REC floats, or null
   float qavg[HVM_ACTIONS];       // learned mode: the network's outputs averaged over the turns
   uint32_t qn;                   // of this normalisation (exponentially), and their count
-  HvmTurnStats turn[HVM_STAT_STRIPES];   // summed by the between kernel
+  HvmTurnStats prev;             // the evaluator's counters, summed, at the previous decision
 };
+
+// The scheduler's view of summed counters (host and device).
+__host__ __device__ inline void hvm_turn_view(const HvmDevStats& x, HvmTurnStats& v) {
+  v.itrs = x.itrs;
+  v.wave_steps = x.steps;
+  v.lane_steps = x.lane_steps;
+  v.spills = x.spills;
+  v.defers = x.defers;
+  v.probes = x.probes;
+  for (int k = 0; k < HVM_K_KINDS; ++k) v.kind[k] = x.kind[k];
+  for (int k = 0; k < 4; ++k) v.wave_left[k] = x.wave_left[k];
+  v.alloc_slots = x.alloc_slots;
+}
 
 struct HvmTurnArgs {
   // net

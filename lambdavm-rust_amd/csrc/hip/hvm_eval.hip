@@ -1439,20 +1439,12 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
 #pragma unroll
         for (int k = 0; k < 6; ++k) if (ph[k]) atomicAdd(&St->phase[k], ph[k]);
       }
-      A.wave_state[w] = ws;
-      // per-turn counters for the scheduler
-      HvmTurnStats* T = &A.sched->turn[w & (HVM_STAT_STRIPES - 1)];
-      if (s_itrs) atomicAdd(&T->itrs, s_itrs);
-      if (steps) atomicAdd(&T->wave_steps, steps);
-      if (lane_steps) atomicAdd(&T->lane_steps, lane_steps);
-      if (s_spill) atomicAdd(&T->spills, s_spill);
-      if (defers) atomicAdd(&T->defers, defers);
-      if (probes) atomicAdd(&T->probes, probes);
-#pragma unroll
-      for (int k = 0; k < HVM_K_KINDS; ++k) if (kinds[k]) atomicAdd(&T->kind[k], kinds[k]);
+      // for the scheduler's features
+      if (lane_steps) atomicAdd(&St->lane_steps, lane_steps);
       const u32 cls = left_tot == 0 ? 0u : left_tot <= HVM_WAVE ? 1u : left_tot <= 2u * HVM_WAVE ? 2u : 3u;
-      atomicAdd(&T->wave_left[cls], 1ull);
-      if (alloc_slots) atomicAdd(&T->alloc_slots, alloc_slots);
+      atomicAdd(&St->wave_left[cls], 1ull);
+      A.wave_state[w] = ws;
+      if (alloc_slots) atomicAdd(&St->alloc_slots, alloc_slots);
     }
   }
 }
@@ -1615,31 +1607,32 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
   HvmCtl* c = A.ctl;
   HvmSched* s = A.sched;
   const u32 t = threadIdx.x;
-  // the turn's counters: lane t sums copies t, t + 64, ... into tsum, and
-  // clears them
-  static_assert(HVM_STAT_STRIPES % HVM_WAVE == 0, "copies of the turn counters: a multiple of the lanes");
-  static_assert(sizeof(HvmTurnStats) % 8 == 0, "turn counters are u64");
+  // the turn's counters: the evaluator's counters summed over their copies
+  // (lane t sums copies t, t + 64, ..., every load in flight at once), minus
+  // their sums at the previous decision
+  static_assert(HVM_STAT_STRIPES % HVM_WAVE == 0, "copies of the counters: a multiple of the lanes");
+  static_assert(sizeof(HvmTurnStats) % 8 == 0 && sizeof(HvmTurnStats) / 8 <= HVM_WAVE, "turn counters: u64, one per lane");
   {
-    // every load in flight at once, then the clears, then the wave sums
     constexpr u32 NF = sizeof(HvmTurnStats) / 8;
     unsigned long long v[NF];
 #pragma unroll
     for (u32 k = 0; k < NF; ++k) v[k] = 0;
     for (u32 j = t; j < HVM_STAT_STRIPES; j += HVM_WAVE) {
-      const unsigned long long* mine = (const unsigned long long*)&s->turn[j];
+      HvmTurnStats x;
+      hvm_turn_view(A.stats[j], x);
+      const unsigned long long* xv = (const unsigned long long*)&x;
 #pragma unroll
-      for (u32 k = 0; k < NF; ++k) v[k] += mine[k];
-    }
-    for (u32 j = t; j < HVM_STAT_STRIPES; j += HVM_WAVE) {
-      unsigned long long* mine = (unsigned long long*)&s->turn[j];
-#pragma unroll
-      for (u32 k = 0; k < NF; ++k) mine[k] = 0;
+      for (u32 k = 0; k < NF; ++k) v[k] += xv[k];
     }
     unsigned long long* sum = (unsigned long long*)&tsum;
+    unsigned long long* prev = (unsigned long long*)&s->prev;
 #pragma unroll
     for (u32 k = 0; k < NF; ++k) {
       const unsigned long long x = wave_sum64(v[k]);
-      if (t == 0) sum[k] = x;
+      if (t == k) {
+        sum[k] = x - prev[k];
+        prev[k] = x;
+      }
     }
   }
   __syncthreads();

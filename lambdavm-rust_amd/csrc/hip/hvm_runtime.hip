@@ -409,7 +409,14 @@ struct hvm_hip_ctx {
     s.trace = d_trace;
     memset(s.qavg, 0, sizeof s.qavg);
     s.qn = 0;
-    memset(&s.turn, 0, sizeof s.turn);
+    // the counters' sums so far (a later normalisation of the same
+    // evaluation, or a later group of a batch, continues the totals)
+    memset(&s.prev, 0, sizeof s.prev);
+    if (!new_eval) {
+      HvmDevStats st;
+      if (!read_stats(st)) return false;
+      hvm_turn_view(st, s.prev);
+    }
     return ck(hipMemcpy(sched, &s, sizeof s, hipMemcpyHostToDevice), "sched upload");
   }
 
