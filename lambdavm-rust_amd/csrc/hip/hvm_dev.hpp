@@ -65,7 +65,9 @@ struct HvmDevDef {
 // The def record the evaluator loads for a CALL: the header and the first
 // template pairs in one 128-byte record, so that one round trip brings them.
 // Further pairs are read from the flat pair array.
+#ifndef HVM_PRE_NODE
 #define HVM_PRE_NODE 8
+#endif
 #define HVM_PRE_RBAG 4
 struct HvmDevDefX {
   HvmDevDef h;
