@@ -264,6 +264,12 @@ int hvm_hip_request_rate(uint64_t slots, uint32_t bytes, uint32_t op, double* op
  * the best of 5 device-to-device copies of `bytes` (1 MiB..8 GiB). */
 int hvm_hip_copy_rate(uint64_t bytes, double* bytes_per_s);
 
+/* Self-test of the evaluator's cross-lane primitives (wave prefix sum, max,
+ * 64-bit sum) on n_waves * 64 inputs; per lane 7 words: exclusive prefix
+ * sum, wave sum, wave max, the same prefix sum and wave sum over LDS
+ * permutes, and the 64-bit wave sum (low, high). */
+int hvm_hip_wave_selftest(uint32_t n_waves, const uint32_t* in32, const uint64_t* in64, uint32_t* out);
+
 /* ------------------------------------------------------------------------ */
 /* Front end (the reference's Rust host, restated in C++)                   */
 /* ------------------------------------------------------------------------ */

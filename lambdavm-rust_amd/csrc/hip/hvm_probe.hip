@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "../../../include/hvm_hip.h"
+#include "hvm_wave.hpp"
 
 namespace {
 
@@ -75,7 +76,45 @@ int run_probe(uint64_t slots, double* ops_per_s) {
   return rc;
 }
 
+// The evaluator's wave primitives on n waves of inputs; per lane 7 words:
+// exclusive scan, its total, max, the scan and total over LDS permutes, and
+// the 64-bit sum (low, high).
+__global__ __launch_bounds__(HVM_WAVE) void wave_selftest_kernel(const uint32_t* in32, const unsigned long long* in64,
+                                                                  uint32_t* out) {
+  const uint32_t i = blockIdx.x * HVM_WAVE + threadIdx.x;
+  const uint32_t x = in32[i];
+  uint32_t tot = 0, tot_ref = 0;
+  const uint32_t sc = wave_scan(x, &tot);
+  const uint32_t mx = wave_max(x);
+  const uint32_t sc_ref = wave_scan_shfl(x, &tot_ref);
+  const unsigned long long s64 = wave_sum64(in64[i]);
+  uint32_t* o = out + (size_t)i * 7;
+  o[0] = sc; o[1] = tot; o[2] = mx; o[3] = sc_ref; o[4] = tot_ref;
+  o[5] = (uint32_t)s64; o[6] = (uint32_t)(s64 >> 32);
+}
+
 }  // namespace
+
+extern "C" int hvm_hip_wave_selftest(uint32_t n_waves, const uint32_t* in32, const uint64_t* in64, uint32_t* out) {
+  if (!in32 || !in64 || !out || n_waves == 0 || n_waves > 4096) return HVM_HIP_E_ARG;
+  const size_t n = (size_t)n_waves * HVM_WAVE;
+  uint32_t* d32 = nullptr;
+  unsigned long long* d64 = nullptr;
+  uint32_t* dout = nullptr;
+  int rc = HVM_HIP_E_DEVICE;
+  if (hipMalloc(&d32, n * 4) == hipSuccess && hipMalloc(&d64, n * 8) == hipSuccess &&
+      hipMalloc(&dout, n * 7 * 4) == hipSuccess && hipMemcpy(d32, in32, n * 4, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(d64, in64, n * 8, hipMemcpyHostToDevice) == hipSuccess) {
+    hipLaunchKernelGGL(wave_selftest_kernel, dim3(n_waves), dim3(HVM_WAVE), 0, 0, d32, d64, dout);
+    if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(out, dout, n * 7 * 4, hipMemcpyDeviceToHost) == hipSuccess)
+      rc = HVM_HIP_OK;
+  }
+  (void)hipFree(d32);
+  (void)hipFree(d64);
+  (void)hipFree(dout);
+  return rc;
+}
 
 extern "C" int hvm_hip_copy_rate(uint64_t bytes, double* bytes_per_s) {
   if (!bytes_per_s || bytes < (1u << 20) || bytes > (8ull << 30)) return HVM_HIP_E_ARG;

@@ -368,6 +368,23 @@ def copy_rate(nbytes: int) -> float:
     return out.value
 
 
+def wave_selftest(in32, in64):
+    """hvm_hip_wave_selftest on len(in32) // 64 waves: per lane (scan, total,
+    max, scan over LDS permutes, their total, 64-bit sum)."""
+    n = len(in32)
+    assert n % 64 == 0 and len(in64) == n
+    L = lib()
+    L.hvm_hip_wave_selftest.argtypes = [C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint32)]
+    out = (C.c_uint32 * (n * 7))()
+    rc = L.hvm_hip_wave_selftest(n // 64, (C.c_uint32 * n)(*in32), (C.c_uint64 * n)(*in64), out)
+    if rc != 0:
+        raise HvmError(f"hvm_hip_wave_selftest failed: {STATUS.get(rc, rc)}")
+    o = list(out)
+    return [(o[7 * i], o[7 * i + 1], o[7 * i + 2], o[7 * i + 3], o[7 * i + 4], o[7 * i + 5] | (o[7 * i + 6] << 32))
+            for i in range(n)]
+
+
 def request_rate(slots: int, nbytes: int, op: int) -> float:
     """Device-scope requests per second on random slots (hvm_hip_request_rate):
     op 0 exchanges, 1 stores, 2 loads."""
