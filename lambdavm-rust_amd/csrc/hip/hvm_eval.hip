@@ -252,7 +252,7 @@ __device__ __forceinline__ void push_redex(WaveLds& L, LocalNet* N, Lane& ln, co
       const u32 leader = (u32)__ffsll((long long)m) - 1u;
       u32 base = 0;
       if (__lane_id() == leader) base = atomicAdd(&A.ctl->n_out, (u32)__popcll(m));
-      const u32 pos = __shfl(base, (int)leader, HVM_WAVE) + rank_in(m);
+      const u32 pos = lane_u32(base, (int)leader) + rank_in(m);
       if (pos < A.bag_cap) A.bag[A.ctl->parity ^ 1u][pos] = old;
       else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     }
@@ -737,7 +737,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         u32 k = __popcll(em);
         u32 base = 0;
         if (lane == 0) base = atomicAdd(&A.ctl->in_head, k);
-        base = __shfl(base, 0, HVM_WAVE);
+        base = lane_u32(base, 0);
         if (base >= n_in) more = false;
         u32 r = rank_in(em);
         if (empty && base + r < n_in) push_redex<LOC>(L, N, ln, A, bag_in[base + r]);
@@ -1330,7 +1330,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         u32 k = __popcll(om);
         u32 base = 0;
         if (lane == 0) base = atomicAdd(&A.ctl->n_out, k);
-        base = __shfl(base, 0, HVM_WAVE);
+        base = lane_u32(base, 0);
         u32 pos = base + rank_in(om);
         if (out1) {
           if (pos < A.bag_cap) bag_out[pos] = emit;
@@ -1366,7 +1366,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     left_tot = tot;
     u32 base = 0;
     if (lane == 0 && tot) base = atomicAdd(&A.ctl->n_out, tot);
-    base = __shfl(base, 0, HVM_WAVE);
+    base = lane_u32(base, 0);
     if (base + tot > A.bag_cap) {
       if (lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     } else {
@@ -1387,12 +1387,12 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   while (more) {
     u32 base = 0;
     if (lane == 0) base = atomicAdd(&A.ctl->in_head, (u32)HVM_WAVE);
-    base = __shfl(base, 0, HVM_WAVE);
+    base = lane_u32(base, 0);
     if (base >= n_in) break;
     u32 cnt = min((u32)HVM_WAVE, n_in - base);
     u32 obase = 0;
     if (lane == 0) obase = atomicAdd(&A.ctl->n_out, cnt);
-    obase = __shfl(obase, 0, HVM_WAVE);
+    obase = lane_u32(obase, 0);
     if (obase + cnt > A.bag_cap) {
       if (lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
       break;

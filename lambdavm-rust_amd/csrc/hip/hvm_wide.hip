@@ -84,7 +84,7 @@ __device__ inline void spill(const HvmWideArgs& A, u64 a, u64 b) {
   const u32 leader = (u32)__ffsll((long long)m) - 1u;
   u32 base = 0;
   if (lane_id() == leader) base = atomicAdd(&A.ctl->n_out, (u32)__popcll(m));
-  const u32 pos = __shfl(base, (int)leader, HVM_WAVE) + rank_in(m);
+  const u32 pos = lane_u32(base, (int)leader) + rank_in(m);
   if (pos < A.bag_cap) {
     u64* out = A.bag[A.ctl->parity ^ 1u];
     out[2ull * pos] = a;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
         const u32 k = (u32)__popcll(em);
         u32 base = 0;
         if (lane == 0) base = atomicAdd(&C->in_head, k) + HVM_WAVE * A.nwaves;
-        base = __shfl(base, 0, HVM_WAVE);
+        base = lane_u32(base, 0);
         if (base >= n_in) more = false;
         const u32 r = rank_in(em);
         if (empty && base + r < n_in) push(L, ln, A, bin[2ull * (base + r)], bin[2ull * (base + r) + 1]);
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_wide_turn_kernel(HvmWideArgs A) 
   while (more) {
     u32 base = 0;
     if (lane == 0) base = atomicAdd(&C->in_head, (u32)HVM_WAVE) + HVM_WAVE * A.nwaves;
-    base = __shfl(base, 0, HVM_WAVE);
+    base = lane_u32(base, 0);
     if (base >= n_in) break;
     if (base + lane < n_in) spill(A, bin[2ull * (base + lane)], bin[2ull * (base + lane) + 1]);
   }
