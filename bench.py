@@ -14,7 +14,7 @@ record) is checked on the first warmup step, and ITRS on every timed step.
 
 Both schedulers are timed with the same protocol (W warmup + K steps each):
 the plain heuristic and the learned MLP; `value` is the scheduler chosen by
---sched (plain by default).  The 256-net batched workload is timed the same
+--sched (learned by default).  The 256-net batched workload is timed the same
 way for both schedulers.  After the timed regions: the bitonic sort of 2^16
 keys (with its own roofline) and the trie radix sort of 2^18 distinct keys.
 
