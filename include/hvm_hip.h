@@ -193,12 +193,15 @@ int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_net
 /* ------------------------------------------------------------------------ */
 
 /* After every turn the scheduler picks how the next turn shares work: one of
- * 4 actions (wave share, idle-quit share) = (all, idle_pct), (all, 50%),
- * (1/2, idle_pct), (all, 10%).  A turn runs on that share of the waves
- * (the others end at once; fewer, fuller waves for a small frontier) and ends
- * once its idle-quit share of them has run out of work, when the HBM bag
- * redistributes the redexes; turns are at most 4096 steps.  idle_pct is
- * hvm_hip_cfg.idle_pct (25% by default).  A turn on part of the waves
+ * 4 actions (wave share, idle-quit share, steady bound) = (all, idle_pct, 1),
+ * (all, 50%, 1), (1/2, idle_pct, 1), (all, idle_pct, 16).  A turn runs on
+ * that share of the waves (the others end at once; fewer, fuller waves for a
+ * small frontier) and ends once its idle-quit share of them has run out of
+ * work, when the HBM bag redistributes the redexes; turns are at most 4096
+ * steps.  idle_pct is hvm_hip_cfg.idle_pct (25% by default).  A turn that
+ * follows one that did not grow the redexes by more than half, and left at
+ * most (steady bound)/4 of them per wave, runs without idle quit until some
+ * wave holds more than two redexes per lane (steady chains).  A turn on part of the waves
  * allocates from those waves' slices and the shared pool only.  The choice
  * never changes what an interaction does, so normal form and ITRS do not
  * depend on it.
@@ -208,9 +211,11 @@ int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_net
  *                         from the finished turn's features (bag sizes, lane
  *                         use, interaction mix, how the turn ended, per-wave
  *                         occupancy at its end, free-slot density, rate);
- *                         it replaces action 0 only when its predicted
- *                         value is higher by more than explore_pct/100
- *                         (the plain choice stays the fallback)
+ *                         it moves off the action in force (action 0
+ *                         when a normalisation starts) only to an action
+ *                         whose predicted value is higher by more than
+ *                         explore_pct/100 (the plain choice stays the
+ *                         fallback; near-ties keep the action in force)
  *   HVM_HIP_SCHED_EXPLORE action 0 with explore_pct% random choices
  *                         (seeded), for collecting training traces
  *   HVM_HIP_SCHED_FIXED   action explore_pct % 4, every turn

@@ -158,7 +158,7 @@ struct HvmCtl {
 // an interaction does, only when redexes are shared.
 // ---------------------------------------------------------------------------
 
-#define HVM_ACTIONS     4      // (wave share, idle-quit share) pairs the scheduler chooses from
+#define HVM_ACTIONS     4      // (wave share, idle-quit share, steady bound) triples the scheduler chooses from
 #define HVM_FEAT        32     // MLP input features
 #define HVM_HID         64     // MLP hidden width
 // packed fp16 parameters: W1[32][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4]

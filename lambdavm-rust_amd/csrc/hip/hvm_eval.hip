@@ -31,6 +31,14 @@ typedef uint32_t u32;
 
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
 
+// Steps between a busy wave's looks at the idle-quit count: a power of two
+// of at most 64, so that the looks include every 64th step, where the wave
+// also reads the error flag.
+#ifndef HVM_QUIT_EVERY
+#define HVM_QUIT_EVERY 16u
+#endif
+static_assert((HVM_QUIT_EVERY & (HVM_QUIT_EVERY - 1u)) == 0 && HVM_QUIT_EVERY <= 64u, "HVM_QUIT_EVERY");
+
 // ---------------------------------------------------------------------------
 // Net accessors: every shared node/var access is a device-scope atomic.
 // ---------------------------------------------------------------------------
@@ -772,7 +780,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       if (lane == 0) atomicAdd(&A.ctl->idle_waves, 1u);
       break;
     }
-    if ((step & 15) == 15) {
+    if ((step & (HVM_QUIT_EVERY - 1u)) == HVM_QUIT_EVERY - 1u) {
       if (!evac) {
         // A turn that started without idle quit (few redexes, not growing:
         // independent sequential chains) gets it back once some wave holds
@@ -1524,22 +1532,29 @@ __global__ __launch_bounds__(256) void hvm_gather_kernel(HvmGatherArgs G) {
 // all 64 lanes evaluate the MLP in learned mode.
 // ---------------------------------------------------------------------------
 
-// Scheduler actions: the share of the waves a turn runs on (1/div) and its
-// idle-quit share (0: the configured one, hvm_hip_cfg.idle_pct); turns are at
-// most 4096 steps.  Action 0 is the plain heuristic's.  Fewer waves put a
-// small frontier's redexes on fewer, fuller waves; a larger idle-quit share
-// lets waves run longer before the HBM bag redistributes.  (Shorter fixed
-// turns, 512 and 128 steps, were slower than idle quitting on every workload
-// measured, by up to 19%.)
+// Scheduler actions: the share of the waves a turn runs on (1/div), its
+// idle-quit share (0: the configured one, hvm_hip_cfg.idle_pct) and the
+// factor on the redex count below which a turn that did not grow makes the
+// next one steady (the steady rule below); turns are at most 4096 steps.
+// Action 0 is the plain heuristic's.  Fewer waves put a small frontier's
+// redexes on fewer, fuller waves; a larger idle-quit share lets waves run
+// longer before the HBM bag redistributes; a 16 times larger steady bound
+// (four redexes per wave) lets a frontier that narrows towards a join run
+// its chains without turn ends.  (Shorter fixed turns, 512 and 128 steps,
+// were slower than idle quitting on every workload measured, by up to 19%.)
 #ifndef HVM_SCHED_WDIV
 #define HVM_SCHED_WDIV 1, 1, 2, 1
 #endif
 #ifndef HVM_SCHED_IDLE
-#define HVM_SCHED_IDLE 0, 50, 0, 10
+#define HVM_SCHED_IDLE 0, 50, 0, 0
+#endif
+#ifndef HVM_SCHED_STEADY
+#define HVM_SCHED_STEADY 1, 1, 1, 16
 #endif
 #define HVM_TURN_LEN 4096u
 __device__ const u32 kWaveDiv[HVM_ACTIONS] = {HVM_SCHED_WDIV};
 __device__ const u32 kIdlePct[HVM_ACTIONS] = {HVM_SCHED_IDLE};
+__device__ const u32 kSteadyMul[HVM_ACTIONS] = {HVM_SCHED_STEADY};
 
 // The plain heuristic: every wave, the longest turn, ended early once the
 // configured idle-quit share of the waves has run out of work, which
@@ -1551,9 +1566,10 @@ __host__ __device__ __forceinline__ u32 plain_action(u32 n, u32 nwaves) {
 }
 
 // The steady rule: after a turn that ends with at most NUM/DEN redexes per
-// wave and at most GROWTH_PCT% of those it started with, the next turn runs
-// without idle quit (waves run their chains until they run dry); idle quit
-// comes back inside the turn once a wave holds surplus work.
+// wave (times the action's factor) and at most GROWTH_PCT% of those it
+// started with, the next turn runs without idle quit (waves run their chains
+// until they run dry); idle quit comes back inside the turn once a wave holds
+// surplus work.
 #ifndef HVM_STEADY_NUM
 #define HVM_STEADY_NUM 1u
 #endif
@@ -1602,7 +1618,7 @@ __device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__u
 
 __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u32 clock_khz) {
   __shared__ float f[HVM_FEAT], h1[HVM_HID], h2[HVM_HID], y[HVM_ACTIONS];
-  __shared__ u32 s_go, s_n, s_steady;
+  __shared__ u32 s_go, s_n, s_flat;
   __shared__ HvmTurnStats tsum;
   HvmCtl* c = A.ctl;
   HvmSched* s = A.sched;
@@ -1638,7 +1654,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
   __syncthreads();
   if (t == 0) {
     s_go = 0;
-    s_steady = 0;
+    s_flat = 0;
     if (!c->done) {
       u32 n = c->n_out;
       c->turns += 1;
@@ -1661,10 +1677,10 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
           rec[HVM_FEAT + 3] = dt_us;
         }
         sched_features(c, s, tsum, n, A.nwaves, dt_us, f);
-        // few redexes, not growing: independent sequential chains, which a
-        // redistribution cannot spread; the next turn runs without idle quit
-        s_steady = (u64)n * HVM_STEADY_DEN <= (u64)A.nwaves * HVM_STEADY_NUM &&
-                   (u64)n * 100u <= (u64)c->n_in * HVM_STEADY_GROWTH_PCT;
+        // not growing: with few redexes (checked once the action is known),
+        // independent sequential chains, which a redistribution cannot
+        // spread; the next turn then runs without idle quit
+        s_flat = (u64)n * 100u <= (u64)c->n_in * HVM_STEADY_GROWTH_PCT;
         c->n_in = n;
         c->n_out = 0;
         c->idle_waves = 0;
@@ -1712,12 +1728,15 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
     }
     __syncthreads();
     if (t == 0) s->qn += 1;
-    // The plain heuristic is the fallback: the network overrides its choice
-    // only when it predicts a whole-evaluation speedup larger than the margin
-    // (explore_pct / 100, in log2 units).
+    // The plain heuristic is the fallback: the network moves off the action
+    // in force (the plain one when a normalisation starts) only to an action
+    // whose predicted whole-evaluation speedup beats it by more than the
+    // margin (explore_pct / 100, in log2 units), so that near-ties do not
+    // alternate actions from turn to turn.
+    const u32 cur = s->last_action < HVM_ACTIONS ? s->last_action : action;
     u32 best = 0;
     for (u32 k = 1; k < HVM_ACTIONS; ++k) if (y[k] > y[best]) best = k;
-    if (y[best] > y[action] + 0.01f * (float)s->explore_pct) action = best;
+    action = y[best] > y[cur] + 0.01f * (float)s->explore_pct ? best : cur;
   } else if (s->mode == HVM_SCHED_EXPLORE) {
     const u32 h = sched_hash(s->seed, c->turns);
     if (h % 100u < s->explore_pct) action = (h >> 8) & 3u;
@@ -1736,7 +1755,8 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
     c->in_head = min(n, act * (u32)HVM_WAVE);
     c->idle_back = back;
     c->idle_quit = kIdlePct[action] ? max(1u, (act * kIdlePct[action]) / 100u) : back;
-    if (s_steady) c->idle_quit = 0;
+    if (s_flat && (u64)n * HVM_STEADY_DEN <= (u64)A.nwaves * HVM_STEADY_NUM * kSteadyMul[action])
+      c->idle_quit = 0;
     s->last_action = action;
     if (s->trace && s->trace_len < s->trace_cap) {
       float* rec = s->trace + (size_t)s->trace_len * HVM_TRACE_REC;

@@ -1,11 +1,13 @@
 """Learned scheduler: trace collection, training and the A/B test.
 
 After every turn the evaluator's scheduler chooses how the next turn shares
-work, one of 4 (wave share, idle-quit share) actions: (all, the configured
-25%), (all, 50%), (half, 25%), (all, 10%).  A turn runs on that share of the
-waves and ends once its idle-quit share of them has run dry; then the HBM bag
-redistributes the redexes.  The plain heuristic always takes action 0 (with
-the steady-chain and surplus rules of the turn kernel).  The learned
+work, one of 4 (wave share, idle-quit share, steady bound) actions: (all,
+the configured 25%, 1), (all, 50%, 1), (half, 25%, 1), (all, 25%, 16).  A turn
+runs on that share of the waves and ends once its idle-quit share of them has
+run dry; then the HBM bag redistributes the redexes.  After a turn that did
+not grow the redexes and left at most (steady bound)/4 per wave, the next
+turn runs without idle quit (steady chains) until a wave holds surplus work.
+The plain heuristic always takes action 0.  The learned
 scheduler is a 32->64->64->4 MLP (fp16 weights, fp32 accumulation) that reads
 the finished turn's features — pending redexes, lane occupancy, per-kind
 interaction mix, spill/defer/probe rates, how the turn ended (waves run dry,
