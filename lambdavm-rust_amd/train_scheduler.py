@@ -1,7 +1,7 @@
 """Trains the learned scheduler on the GPU and writes its weights.
 
     python lambdavm-rust_amd/train_scheduler.py [--n 12 --m 12] [--train-seeds 64]
-        [--also radix_lsd,batch,stress,bitonic,church,sum,radix] [--also-seeds 8]
+        [--also radix_lsd,batch,stress,stress_18,bitonic,church,sum,radix] [--also-seeds 8]
         [--eval-runs 12] [--out lambdavm-rust_amd/sched_weights.json]
 
 Workload: church_loop(n, m) of programs.py (an outer loop of 2^n iterations,
@@ -9,9 +9,9 @@ each applying the Church numeral 2^m to a u24 increment), evaluated at seeds
 0..train_seeds-1 once under each of the 4 actions held fixed for every turn;
 `--also` adds other workloads the same way at seeds 0..also_seeds-1
 (radix_lsd: the LSD radix sort of 2^18 keys; batch: the 256-net batch of
-programs.batch(0); stress: the reference's stress program with 2^16 loops of
-256; bitonic, radix, sum, church: the earlier sort, sum and Church-fan
-programs), so that one network serves them all.  Every recorded turn of a
+programs.batch(0); stress, stress_18: the reference's stress program with 2^16
+loops of 256 and 2^18 loops of 1024; bitonic, radix, sum, church: the earlier
+sort, sum and Church-fan programs), so that one network serves them all.  Every recorded turn of a
 workload is a training input, and its targets are the whole-evaluation
 speedups of the 4 actions on that workload (log2 of median time under
 action 0 over median time under the action).  The MLP is fitted with the
@@ -47,8 +47,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=800)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--eval-runs", type=int, default=12)
-    ap.add_argument("--also", default="radix_lsd,batch,stress,bitonic,church,sum,radix",
-                    help="more workloads to train on: radix_lsd,batch,stress,bitonic,radix,sum,church")
+    ap.add_argument("--also", default="radix_lsd,batch,stress,stress_18,bitonic,church,sum,radix",
+                    help="more workloads to train on: radix_lsd,batch,stress,stress_18,bitonic,radix,sum,church")
     ap.add_argument("--also-seeds", type=int, default=8)
     ap.add_argument("--unseen", default="church_loop_11,stress_14",
                     help="programs evaluated but never trained on: church_loop_11 (church_loop(11, 12)), "
@@ -85,6 +85,7 @@ def main():
                 "church": ("church(12, 12)", lambda: P.church(12, 12)),
                 "stress": ("stress(16, 256)", lambda: P.stress(16, 256)),
                 "stress_14": ("stress(14, 256)", lambda: P.stress(14, 256)),
+                "stress_18": ("stress(18, 1024)", lambda: P.stress(18, 1024)),
                 "church_loop_11": ("church_loop(11, 12)", lambda: P.church_loop(11, 12))}
         label, gen = gens[name]
         code, expect = gen()

@@ -142,20 +142,20 @@ def _ab(hip, ctx, run, seeds, w):
 
 def test_trained_scheduler_gain(hip):
     """The committed scheduler weights (lambdavm-rust_amd/sched_weights.json:
-    trained on church_loop(12, 12) at seeds 0-63 and on seven other workloads
+    trained on church_loop(12, 12) at seeds 0-63 and on eight other workloads
     at seeds 0-7, each under every action held fixed) against the plain
     heuristic at held-out seeds 100-109, with identical normal form and ITRS
     in every run:
     * church_loop(12, 12), the outer-loop Church workload: learned at least
-      1% faster (median time; measured 1.025x by train_scheduler.py at
+      1% faster (median time; measured 1.028x by train_scheduler.py at
       seeds 64-75: it runs its turns with a 50% idle-quit share or on half of
       the waves, worth 1.035x and 1.02x when held fixed);
     * radix_lsd(18, 0), the LSD radix sort: learned at least 1% faster
-      (measured 1.024x at seeds 8-19: it takes the 16 times larger steady
+      (measured 1.023x at seeds 8-19: it takes the 16 times larger steady
       bound, worth 1.025x when held fixed, which lets the narrowing merges
       run their chains without turn ends);
     * the 256-net batch: learned not slower than plain by more than 0.5%
-      (measured 1.002x at seeds 8-19: no action held fixed beats plain by
+      (measured 1.001x at seeds 8-19: no action held fixed beats plain by
       more than 0.1% there)."""
     path = os.path.join(ROOT, "lambdavm-rust_amd", "sched_weights.json")
     w = json.load(open(path))
