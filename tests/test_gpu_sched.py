@@ -225,12 +225,12 @@ def test_autotuner_keeps_results(hip):
     assert not any("flags" in m.get("error", "") for m in rep["measurements"])
 
 
-@pytest.mark.parametrize("name", ["radix_lsd", "batch"])
+@pytest.mark.parametrize("name", ["radix_lsd", "batch", "church_loop"])
 def test_tuned_configurations_keep_parity(hip, name):
-    """The autotuner's choice for the LSD radix sort of 2^18 keys and for the
-    256-net batch (lambdavm-rust_amd/tuned_<name>.json), compile-time variant
-    included, evaluated in a child process that loads that build: ITRS and
-    normal forms equal the run-c records."""
+    """The autotuner's choice for the LSD radix sort of 2^18 keys, the 256-net
+    batch and church_loop(12, 12) (lambdavm-rust_amd/tuned_<name>.json),
+    compile-time variant included, evaluated in a child process that loads
+    that build: ITRS and normal forms equal the run-c records."""
     import subprocess
     import sys
     path = os.path.join(ROOT, "lambdavm-rust_amd", f"tuned_{name}.json")
@@ -255,5 +255,5 @@ def test_tuned_configurations_keep_parity(hip, name):
         assert res["nets_nf_sha256"] == [row["nf_sha256"] for row in g["nets"]]
     else:
         g = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
-        case = next(c for c in g["workloads"] if c["generator"] == "radix_lsd" and c.get("full_size"))
+        case = next(c for c in g["workloads"] if c["generator"] == name and c.get("full_size"))
         assert res["ident"] == [case["itrs"], case["nf_sha256"]]
