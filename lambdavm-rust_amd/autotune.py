@@ -24,7 +24,8 @@ variants`), each measured in a child process that loads it (HVM_HIP_LIB):
                    scan4 / scan32: slice windows an allocation scans before
                    the shared pool (4 / 32 instead of 8); lnet512 / lnet2048:
                    local-net sizes of the local-net evaluator (1024 default),
-                   tried only when that evaluator is selected.
+                   tried only when that evaluator is selected; pool: the
+                   mid-turn redex pool (HVM_MID_POOL).
 The kernel runs one wave per workgroup; waves per workgroup is not a knob.
 
 Search: coordinate descent from the defaults, one knob at a time, keeping a
@@ -84,7 +85,7 @@ def batched(seed: int = 0, n: int = 256) -> Workload:
     return w
 
 
-VARIANTS = ("deq16", "deq64", "scan4", "scan32", "lnet512", "lnet2048")
+VARIANTS = ("deq16", "deq64", "scan4", "scan32", "lnet512", "lnet2048", "pool")
 
 
 def variant_lib(name: str) -> str:
@@ -150,7 +151,7 @@ def tune(w: Workload, repeats: int = 2, log=print, space=None, base_cfg=None):
         ("turn_steps", [96, 512, 4096, 0]),
         ("flags", [H.COMPILED_CALLS, H.LOCAL_NET, H.LOCAL_NET | H.COMPILED_CALLS, 0]),
         ("graphs", [H.GRAPHS, 0]),
-        ("variant", ["deq16", "deq64", "scan4", "scan32", "lnet512", "lnet2048", ""]),
+        ("variant", ["deq16", "deq64", "scan4", "scan32", "lnet512", "lnet2048", "pool", ""]),
     ]
     best, t_best = dict(base), t_def
     rows = [{"cfg": dict(base), "ms": round(t_def, 3)}]

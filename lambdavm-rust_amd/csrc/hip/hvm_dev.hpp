@@ -89,6 +89,11 @@ struct HvmDevDefX {
 #define HVM_STAT_STRIPES 64
 #endif
 
+// Mid-turn pool of redexes in the output bag (turn kernel); 0: off.
+#ifndef HVM_MID_POOL
+#define HVM_MID_POOL 0
+#endif
+
 // Statistic slots accumulated by the evaluator (same order as the oracle).
 enum {
   HVM_K_CALL = 0, HVM_K_CALL_COPY, HVM_K_VOID, HVM_K_ERAS, HVM_K_ANNI, HVM_K_COMM,
@@ -149,7 +154,8 @@ struct HvmCtl {
   uint32_t surplus;     // some wave of the current turn held more redexes than two per lane
   uint32_t active;      // waves that run the current turn (0: all); the others end at once
   uint32_t idle_back;   // idle quit a turn started without gets back on surplus (0: never)
-  uint32_t pad[2];
+  uint32_t out_take;    // mid-turn pull cursor into the output bag (HVM_MID_POOL)
+  uint32_t pad[1];
 };
 
 // ---------------------------------------------------------------------------

@@ -39,6 +39,31 @@ typedef uint32_t u32;
 #endif
 static_assert((HVM_QUIT_EVERY & (HVM_QUIT_EVERY - 1u)) == 0 && HVM_QUIT_EVERY <= 64u, "HVM_QUIT_EVERY");
 
+// Mid-turn pool (HBM evaluator, HVM_MID_POOL): at its idle-quit looks a busy
+// wave hands the oldest redex of every lane holding three or more to the
+// output bag while fewer than HVM_POOL_LOW handed-over redexes wait there, and
+// a wave that runs dry first claims a wave's worth of output-bag slots (a
+// cursor in HvmCtl) and takes each by an exchange with 0.  Bag slots outside
+// the live range are kept zero for this (every input-bag read is an exchange
+// with 0, and the runtime clears both bags with the heap), so a slot whose
+// store has not landed reads 0 and keeps its redex for the next turn, which
+// skips the zeros.  No wave waits for another.
+#ifndef HVM_POOL_LOW
+#define HVM_POOL_LOW 4096u
+#endif
+// An input-bag slot read by the turn that consumes it (cleared with the mid-turn pool).
+__device__ __forceinline__ u64 bag_take(u64* p) {
+  if (HVM_MID_POOL) return __hip_atomic_exchange(p, 0ull, __ATOMIC_RELAXED, AGENT);
+  return *p;
+}
+// An output-bag write.  With the mid-turn pool it is a device-scope store:
+// a plain store can sit in the writer's L2 and be written back after a
+// pull's exchange took the slot, which would hand the redex out twice.
+__device__ __forceinline__ void bag_put(u64* p, u64 v) {
+  if (HVM_MID_POOL) __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT);
+  else *p = v;
+}
+
 // ---------------------------------------------------------------------------
 // Net accessors: every shared node/var access is a device-scope atomic.
 // ---------------------------------------------------------------------------
@@ -261,7 +286,7 @@ __device__ __forceinline__ void push_redex(WaveLds& L, LocalNet* N, Lane& ln, co
       u32 base = 0;
       if (__lane_id() == leader) base = atomicAdd(&A.ctl->n_out, (u32)__popcll(m));
       const u32 pos = lane_u32(base, (int)leader) + rank_in(m);
-      if (pos < A.bag_cap) A.bag[A.ctl->parity ^ 1u][pos] = old;
+      if (pos < A.bag_cap) bag_put(&A.bag[A.ctl->parity ^ 1u][pos], old);
       else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     }
     ln.spills++;
@@ -644,7 +669,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   const u32 max_steps = C->max_steps;
   const u32 idle_quit = C->idle_quit;
   const u32 idle_back = C->idle_back;
-  const u64* bag_in = A.bag[C->parity];
+  u64* const bag_in = A.bag[C->parity];
   u64* bag_out = A.bag[C->parity ^ 1u];
 
   Lane ln;
@@ -664,6 +689,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
   // (wave-uniform; HBM evaluator with early takes by load only)
   constexpr bool RECYCLE = !LOC && HVM_EARLY_TAKES && HVM_PEEK_TAKES && HVM_NODE_RECYCLE > 0;
   u32 rec_n = 0;
+  constexpr bool POOL = !LOC && HVM_MID_POOL;
 
   if (LOC) {
     // an empty local net: every slot free
@@ -693,7 +719,10 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     const u32 seed = A.sched->seed;
     const u32 rot = seed ? sched_hash(seed, C->turns) % act : 0u;
     u32 idx = (w + rot) % act + lane * act;
-    if (idx < n_in && idx < HVM_WAVE * act) push_redex<LOC>(L, N, ln, A, bag_in[idx]);
+    if (idx < n_in && idx < HVM_WAVE * act) {
+      const u64 x = bag_take(&bag_in[idx]);
+      if (!HVM_MID_POOL || x) push_redex<LOC>(L, N, ln, A, x);   // 0: taken from the pool
+    }
   }
   bool more = n_in > HVM_WAVE * act;
   // With the local net a wave ends its turn by draining the net (evacuation):
@@ -748,7 +777,10 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         base = lane_u32(base, 0);
         if (base >= n_in) more = false;
         u32 r = rank_in(em);
-        if (empty && base + r < n_in) push_redex<LOC>(L, N, ln, A, bag_in[base + r]);
+        if (empty && base + r < n_in) {
+          const u64 x = bag_take(&bag_in[base + r]);
+          if (!HVM_MID_POOL || x) push_redex<LOC>(L, N, ln, A, x);
+        }
       }
     }
 
@@ -775,6 +807,21 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     bool has = ln.top != ln.bot;
     if (!__ballot(has)) {
       if (more && !evac) continue;
+      if (POOL && !evac) {
+        // before counting as idle: up to a wave of the redexes busy waves
+        // handed to the output bag this turn
+        const u32 out_n = __hip_atomic_load(&A.ctl->n_out, __ATOMIC_RELAXED, AGENT);
+        const u32 took = __hip_atomic_load(&A.ctl->out_take, __ATOMIC_RELAXED, AGENT);
+        if (out_n > took) {
+          u32 t = 0;
+          if (lane == 0) t = atomicAdd(&A.ctl->out_take, (u32)HVM_WAVE);
+          t = lane_u32(t, 0);
+          u64 x = 0;
+          if (t + lane < min(out_n, A.bag_cap)) x = __hip_atomic_exchange(&bag_out[t + lane], 0ull, __ATOMIC_RELAXED, AGENT);
+          if (x) push_redex<LOC>(L, N, ln, A, x);
+          if (__ballot(x != 0)) continue;
+        }
+      }
       // out of work: count this wave idle, so that busy waves end the turn
       // early and their redexes get redistributed
       if (lane == 0) atomicAdd(&A.ctl->idle_waves, 1u);
@@ -800,6 +847,28 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         if (iq && idle >= iq) {
           if (!LOC) break;
           evac = true;
+        }
+        if (POOL) {
+          // hand surplus to the pool while it runs low
+          const u32 out_n = __hip_atomic_load(&A.ctl->n_out, __ATOMIC_RELAXED, AGENT);
+          const u32 took = __hip_atomic_load(&A.ctl->out_take, __ATOMIC_RELAXED, AGENT);
+          if (out_n <= took + HVM_POOL_LOW && out_n < A.bag_cap / 2u) {
+            const bool give = ln.top - ln.bot >= 3u;
+            const u64 gm = __ballot(give);
+            if (gm) {
+              u32 base = 0;
+              if (lane == 0) base = atomicAdd(&A.ctl->n_out, (u32)__popcll(gm));
+              const u32 pos = lane_u32(base, 0) + rank_in(gm);
+              if (give) {
+                if (pos < A.bag_cap) {
+                  bag_put(&bag_out[pos], deq_at(L, ln.bot));
+                  ln.bot++;
+                } else {
+                  atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
+                }
+              }
+            }
+          }
         }
       }
       if ((step & 63) == 63 && __hip_atomic_load(&A.ctl->err, __ATOMIC_RELAXED, AGENT) != 0) break;
@@ -1341,7 +1410,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
         base = lane_u32(base, 0);
         u32 pos = base + rank_in(om);
         if (out1) {
-          if (pos < A.bag_cap) bag_out[pos] = emit;
+          if (pos < A.bag_cap) bag_put(&bag_out[pos], emit);
           else atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
         }
       }
@@ -1378,7 +1447,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
     if (base + tot > A.bag_cap) {
       if (lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
     } else {
-      for (u32 i = 0; i < n; ++i) bag_out[base + off + i] = deq_at(L, ln.bot + i);
+      for (u32 i = 0; i < n; ++i) bag_put(&bag_out[base + off + i], deq_at(L, ln.bot + i));
     }
   }
   u64 left = 0;
@@ -1405,7 +1474,7 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       if (lane == 0) atomicOr(&A.ctl->err, (u32)HVM_ERR_BAG_OVERFLOW);
       break;
     }
-    if (lane < cnt) bag_out[obase + lane] = bag_in[base + lane];
+    if (lane < cnt) bag_put(&bag_out[obase + lane], bag_take(&bag_in[base + lane]));
   }
 
   // ---- statistics: one atomic per counter per wave ----
@@ -1683,6 +1752,7 @@ __global__ __launch_bounds__(HVM_WAVE) void hvm_between_kernel(HvmTurnArgs A, u3
         s_flat = (u64)n * 100u <= (u64)c->n_in * HVM_STEADY_GROWTH_PCT;
         c->n_in = n;
         c->n_out = 0;
+        c->out_take = 0;
         c->idle_waves = 0;
         c->surplus = 0;
         c->parity ^= 1u;
@@ -1798,6 +1868,10 @@ extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves) {
 }
 
 extern "C" uint32_t hvm_plain_action_host(uint32_t n, uint32_t nwaves) { return plain_action(n, nwaves); }
+
+// Whether this build's turn kernel runs the mid-turn pool (the runtime then
+// keeps the bags cleared; variant builds recompile only this file).
+extern "C" uint32_t hvm_turn_mid_pool(void) { return HVM_MID_POOL ? 1u : 0u; }
 
 extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s) {
   hipLaunchKernelGGL(hvm_gather_kernel, dim3(1), dim3(256), 0, s, *g);

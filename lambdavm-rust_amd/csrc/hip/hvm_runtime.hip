@@ -24,6 +24,7 @@ extern "C" hipError_t hvm_launch_gather(const HvmGatherArgs* g, hipStream_t s);
 extern "C" hipError_t hvm_launch_between(const HvmTurnArgs* a, uint32_t clock_khz, hipStream_t s);
 extern "C" uint32_t hvm_plain_action_host(uint32_t n, uint32_t nwaves);
 extern "C" uint32_t hvm_turn_policy_host(uint32_t n, uint32_t nwaves);
+extern "C" uint32_t hvm_turn_mid_pool(void);
 extern "C" size_t hvm_turn_lds_bytes(int local);
 extern "C" int hvm_turn_waves_per_cu(int local);
 
@@ -371,6 +372,11 @@ struct hvm_hip_ctx {
     if (!ck(hipMemsetAsync(vars, 0, ((size_t)vars_cap + 1) * 4, stream), "clear vars")) return false;
     if (!ck(hipMemsetAsync(wave_state, 0, nwaves * sizeof(HvmWaveState), stream), "clear cursors")) return false;
     if (!ck(hipMemsetAsync(ctl, 0, sizeof(HvmCtl), stream), "clear ctl")) return false;
+    if (hvm_turn_mid_pool()) {
+      // the mid-turn pool reads a zero slot as one not yet written
+      if (!ck(hipMemsetAsync(bag[0], 0, (size_t)bag_cap * 8, stream), "clear bag")) return false;
+      if (!ck(hipMemsetAsync(bag[1], 0, (size_t)bag_cap * 8, stream), "clear bag")) return false;
+    }
     return ck(hipStreamSynchronize(stream), "reset sync");
   }
 
