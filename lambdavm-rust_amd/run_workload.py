@@ -11,7 +11,10 @@ turns, spills, and the turn kernel's HBM roofline fraction: the algorithmic
 bytes of the interactions (counted by the kernel per interaction kind) per
 dispatch over the average dispatch time (HIP events on the launch stream),
 against 8 TB/s.  With --repeat N the net is evaluated N times and the fastest
-evaluation is reported.  Used for profiles of single workloads.
+evaluation is reported.  With --phases the turn kernel times its step phases
+(HVM_HIP_PROFILE, device wall clock) and the line adds the average wave-step's
+µs per phase: refill+share, classify, takes+alloc, creation, links+count,
+release wait.  Used for profiles of single workloads.
 """
 import argparse
 import json
@@ -35,11 +38,13 @@ def main():
     ap.add_argument("--mode", choices=tuple(MODES), default="interpreted")
     ap.add_argument("--time-limit", type=int, default=60)
     ap.add_argument("--repeat", type=int, default=1)
+    ap.add_argument("--phases", action="store_true", help="time the step phases (HVM_HIP_PROFILE)")
     a = ap.parse_args()
     params = [int(x) for x in a.args.split(",") if x]
     code = getattr(P, a.workload)(*params)[0]
     words = H.compile_book(code)
-    with H.Context(flags=H.NO_READBACK | MODES[a.mode], time_limit_s=a.time_limit) as ctx:
+    flags = H.NO_READBACK | MODES[a.mode] | (H.PROFILE if a.phases else 0)
+    with H.Context(flags=flags, time_limit_s=a.time_limit) as ctx:
         ctx.load(words)
         runs = [ctx.eval() for _ in range(max(1, a.repeat))]
     assert all(x.itrs == runs[0].itrs for x in runs), "ITRS changed between evaluations"
@@ -53,7 +58,9 @@ def main():
                       "waves": r.diag["waves"], "launches": r.launches,
                       "bytes_per_launch": round(r.bytes / max(1, r.launches)),
                       "avg_launch_ms": round(launch_ms / max(1, r.launches), 4),
-                      "hbm_gbs": round(gbs, 2), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+                      "hbm_gbs": round(gbs, 2), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                      **({"us_per_wave_step": [round(ph * 1e3 / max(1, r.diag["steps"]), 3)
+                                               for ph in r.diag["phase_ms"]]} if a.phases else {})}), flush=True)
 
 
 if __name__ == "__main__":
