@@ -201,7 +201,10 @@ int hvm_hip_eval_batch(hvm_hip_ctx* ctx, const uint32_t* entries, uint32_t n_net
  * steps.  idle_pct is hvm_hip_cfg.idle_pct (25% by default).  A turn that
  * follows one that did not grow the redexes by more than half, and left at
  * most (steady bound)/4 of them per wave, runs without idle quit until some
- * wave holds more than two redexes per lane (steady chains).  A turn on part of the waves
+ * wave holds more than two redexes per lane (steady chains); one that
+ * follows a turn that did not grow them and left fewer than there are waves
+ * counts its idle-quit share among the waves that got a redex (the empty
+ * ones count as idle at once).  A turn on part of the waves
  * allocates from those waves' slices and the shared pool only.  The choice
  * never changes what an interaction does, so normal form and ITRS do not
  * depend on it.
