@@ -17,7 +17,8 @@ speedups of the 4 actions on that workload (log2 of median time under
 action 0 over median time under the action).  The MLP is fitted with the
 library's HIP kernels; the margin by which it must beat action 0 is picked on
 training seeds (the best geometric-mean speedup among margins that keep every
-workload within 2% of plain); then plain and learned schedules are timed on alternating
+workload within 2% of plain in the median and within 5% at every seed); then
+plain and learned schedules are timed on alternating
 runs at held-out seeds (which perturb placement and turn lengths in both
 modes) on every workload and on programs the network never saw, and both
 must give the same normal form and ITRS.  The weights file holds the fp16
@@ -55,7 +56,7 @@ def main():
                          "stress_14 (stress with 2^14 loops of 256)")
     ap.add_argument("--margins", type=lambda v: [int(x) for x in v.split(",")], default=[0, 2, 4, 8, 12],
                     help="candidate margins (hundredths of log2 speedup) for the plain fallback")
-    ap.add_argument("--margin-seeds", type=int, default=4, help="training seeds each margin is timed on")
+    ap.add_argument("--margin-seeds", type=int, default=8, help="training seeds each margin is timed on")
     ap.add_argument("--out", default=os.path.join(HERE, "sched_weights.json"))
     args = ap.parse_args()
 
@@ -138,18 +139,23 @@ def main():
     # The margin by which the network must beat action 0 (hundredths of log2
     # speedup) is chosen on training seeds: the best geometric mean of
     # plain/learned median time over the workloads among the margins that
-    # keep every workload within 2% of plain (the plain choice judges any
-    # regression); the largest margin if none does.
-    margins, worst = {}, {}
+    # keep every workload within 2% of plain in the median and within 5% at
+    # every seed (a schedule can go wrong at one placement only: the Church
+    # fan's last turns may run a long sequential fold alone); the largest
+    # margin if none does.
+    margins, worst, seed_worst = {}, {}, {}
     for m in args.margins:
-        rs = []
+        rs, sw = [], []
         for label, spec in progs.items():
             pt, lt, _ = ab(label, spec, range(args.margin_seeds), m)
             rs.append(statistics.median(pt) / statistics.median(lt))
+            sw.append(min(p / q for p, q in zip(pt, lt)))   # pt[i], lt[i]: seed i
         margins[m] = math.exp(statistics.fmean(math.log(r) for r in rs))
         worst[m] = min(rs)
-        print(f"margin {m}: geo-mean speedup on training seeds {margins[m]:.4f} {[round(r, 3) for r in rs]}", flush=True)
-    safe = [m for m in margins if worst[m] >= 0.98]
+        seed_worst[m] = min(sw)
+        print(f"margin {m}: geo-mean speedup on training seeds {margins[m]:.4f} {[round(r, 3) for r in rs]}, "
+              f"worst seed {seed_worst[m]:.3f}", flush=True)
+    safe = [m for m in margins if worst[m] >= 0.98 and seed_worst[m] >= 0.95]
     margin = max(safe, key=margins.get) if safe else max(margins)
 
     evals = {}
@@ -169,7 +175,8 @@ def main():
     first = next(iter(progs))
     report = dict(evals[first], workload=first, train_rows=len(rows), train_seeds=args.train_seeds,
                   loss=loss, eval_runs=args.eval_runs, trained_on=list(progs), per_workload=evals,
-                  margin_pct=margin, margin_search={str(k): v for k, v in margins.items()})
+                  margin_pct=margin, margin_search={str(k): v for k, v in margins.items()},
+                  margin_worst_seed={str(k): v for k, v in seed_worst.items()})
     print(json.dumps({k: v for k, v in report.items() if k != "per_workload"}), flush=True)
     with open(args.out, "w") as f:
         json.dump({"params_fp16": p16, "reward_offset": off, "margin_pct": margin, "report": report}, f)
