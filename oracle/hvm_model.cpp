@@ -77,6 +77,8 @@ typedef struct hvm_model_out {
   uint32_t first_event;    // its kind: 1 zero publish, 2 zero enter, 3 take zero, 4 link cap, 5 clobber
   uint32_t first_err_turn; // turn of the first error flag
   uint64_t first_err_op;   // op index at which the first error flag was raised
+  uint64_t pool_gives;     // redexes handed to the mid-turn pool (HVM_MODEL_POOL)
+  uint64_t pool_takes;     // redexes taken from it by waves that ran dry
   char* result;            // printed normal form of each root, ';'-separated
 } hvm_model_out;
 
@@ -96,6 +98,10 @@ typedef struct hvm_model_out {
 #define HVM_MODEL_VRECYCLE_KEEP 128u // the same mistake on the var stack
 #define HVM_MODEL_ENTER_LOAD 512u // the link loop's ENTER loads B (a build with HVM_ENTER_LOAD;
                                   // else it exchanges NONE into B, as by default)
+#define HVM_MODEL_POOL 1024u     // the mid-turn pool (a build with HVM_MID_POOL): busy waves hand
+                                 // surplus to the output bag at their idle-quit looks, a wave that
+                                 // runs dry takes it back by exchanges with 0; input-bag reads clear
+                                 // their slots, so an unwritten slot reads 0
 #define HVM_MODEL_VRECYCLE 256u  // var slots freed by links go on the wave's var stack (a build
                                   // with HVM_VAR_RECYCLE > 0; off by default, as in the kernel)
 
@@ -171,7 +177,7 @@ struct Model {
   std::vector<u64> bag[2];
   // control block
   u32 n_in = 0, n_out = 0, in_head = 0, err = 0, parity = 0, max_steps = 0, idle_waves = 0, idle_quit = 0;
-  u32 idle_cfg = 0, surplus = 0;
+  u32 idle_cfg = 0, surplus = 0, out_take = 0;
   u32 node_slice = 0, vars_slice = 0, node_pool_lo = 0, vars_pool_lo = 0, node_pool_hi = 0, vars_pool_hi = 0;
   std::vector<WaveState> ws;
   Rng rng;
@@ -514,7 +520,12 @@ void Model::turn(u32 w) {
   if (s.vars_ovf < vars_pool_lo || s.vars_ovf >= vars_pool_hi)
     s.vars_ovf = vars_pool_lo + (u32)(((u64)(vars_pool_hi - vars_pool_lo) * w) / W);
   const u32 nin = n_in;
-  const std::vector<u64>& bag_in = bag[parity];
+  std::vector<u64>& bag_in = bag[parity];
+  std::vector<u64>& bag_out = bag[parity ^ 1u];
+  const bool pool = (C.variant & HVM_MODEL_POOL) != 0;
+  const bool clear_in = pool;
+  // an input-bag read (an exchange with 0 when the pool keeps the bags clear)
+  auto take_in = [&](u32 i) { u64 x = bag_in[i]; if (clear_in) bag_in[i] = 0; return x; };
   // node slots taken by load and not yet handed out again (the kernel's
   // 4 * 64-entry stack, scaled to the model's lanes)
   const bool recycle = early && !(C.variant & HVM_MODEL_NO_RECYCLE);
@@ -522,7 +533,10 @@ void Model::turn(u32 w) {
   std::vector<u32> rec;
   for (u32 l = 0; l < L; ++l) {
     u32 i = w + l * W;
-    if (i < nin && i < L * W) push_redex(lanes[l], bag_in[i], spills);
+    if (i < nin && i < L * W) {
+      u64 x = take_in(i);
+      if (!pool || x) push_redex(lanes[l], x, spills);
+    }
   }
   bool more = nin > L * W;
   std::vector<u32> tbl_n(TB + 8), tbl_v(TB + 8);
@@ -538,7 +552,10 @@ void Model::turn(u32 w) {
         in_head += (u32)empty.size();
         if (base >= nin) more = false;
         for (u32 r = 0; r < empty.size(); ++r)
-          if (base + r < nin) push_redex(lanes[empty[r]], bag_in[base + r], spills);
+          if (base + r < nin) {
+            u64 x = take_in(base + r);
+            if (!pool || x) push_redex(lanes[empty[r]], x, spills);
+          }
       }
     }
     {  // in-wave sharing: needy lanes take the oldest redex of rich lanes
@@ -560,6 +577,25 @@ void Model::turn(u32 w) {
     for (auto& ln : lanes) anyhas |= ln.top != ln.bot;
     if (!anyhas) {
       if (more) continue;
+      if (pool) {
+        // before counting as idle: a wave's worth of handed-over redexes
+        yield();
+        const u32 out_n = n_out;
+        if (out_n > out_take) {
+          const u32 t = out_take;
+          out_take += L;
+          yield();
+          bool got = false;
+          for (u32 r = 0; r < L; ++r) {
+            const u32 i = t + r;
+            if (i >= std::min(out_n, C.bag_cap)) continue;
+            const u64 x = bag_out[i];
+            bag_out[i] = 0;                    // the exchange with 0
+            if (x) { push_redex(lanes[r], x, spills); got = true; O->pool_takes++; }
+          }
+          if (got) continue;
+        }
+      }
       yield();
       idle_waves++;
       break;
@@ -576,6 +612,23 @@ void Model::turn(u32 w) {
         if (surplus) iq = idle_cfg;
       }
       if (iq && idle_waves >= iq) break;
+      if (pool && n_out <= out_take + 4 * L && n_out < C.bag_cap / 2) {
+        // hand the oldest redex of every lane holding three or more to the
+        // pool; the stores land well after the reservation (other waves may
+        // claim and read the slots meanwhile)
+        std::vector<u64> give;
+        for (auto& ln : lanes)
+          if (ln.top - ln.bot >= 3) { give.push_back(ln.deq[ln.bot & (C.deq - 1)]); ln.bot++; }
+        if (!give.empty()) {
+          const u32 base = n_out;
+          n_out += (u32)give.size();
+          for (int k = 0; k < 4; ++k) yield();
+          for (size_t i = 0; i < give.size(); ++i) {
+            if (base + i < C.bag_cap) { bag_out[base + i] = give[i]; O->pool_gives++; }
+            else raise(ERR_BAG);
+          }
+        }
+      }
       if ((step & 63) == 63 && err) break;
     }
     // ---- P0: peek + classify ----
@@ -862,7 +915,7 @@ void Model::turn(u32 w) {
     if (base >= nin) break;
     for (u32 i = base; i < std::min(nin, base + L); ++i) {
       u32 pos = n_out++;
-      if (pos < C.bag_cap) bag[parity ^ 1u][pos] = bag_in[i];
+      if (pos < C.bag_cap) bag[parity ^ 1u][pos] = take_in(i);
       else raise(ERR_BAG);
     }
   }
@@ -1003,6 +1056,7 @@ extern "C" int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_
   m.idle_cfg = idle_cfg;
   for (u32 t = 0; t < (1u << 20); ++t) {
     m.n_out = 0;
+    m.out_take = 0;
     m.idle_waves = 0;
     m.surplus = 0;
     m.run_turn();

@@ -158,6 +158,7 @@ MODEL_RECYCLE_KEEP = 64
 MODEL_VRECYCLE_KEEP = 128
 MODEL_VRECYCLE = 256
 MODEL_ENTER_LOAD = 512
+MODEL_POOL = 1024
 
 
 class _ModelCfg(C.Structure):
@@ -175,7 +176,7 @@ class _ModelOut(C.Structure):
                 ("max_rounds", C.c_uint64), ("full_scans", C.c_uint64), ("pool_claims", C.c_uint64), ("defers", C.c_uint64),
                 ("ops", C.c_uint64), ("first_event_op", C.c_uint64),
                 ("first_event", C.c_uint32), ("first_err_turn", C.c_uint32), ("first_err_op", C.c_uint64),
-                ("result", C.c_void_p)]
+                ("pool_gives", C.c_uint64), ("pool_takes", C.c_uint64), ("result", C.c_void_p)]
 
 
 @dataclass
@@ -227,5 +228,5 @@ def run_model(words, entries=(0,), **cfg) -> ModelResult:
         L.hvm_model_free(o.result)
     ev = {k: getattr(o, k) for k in ("zero_publish", "zero_enter", "take_zero", "putback", "clobber",
                                      "double_alloc", "max_rounds", "full_scans", "pool_claims", "defers", "ops", "first_event_op", "first_event",
-                                     "first_err_op")}
+                                     "first_err_op", "pool_gives", "pool_takes")}
     return ModelResult(rc, o.itrs, {k: o.kind[i] for i, k in enumerate(KINDS)}, o.err, o.turns, ev, res)

@@ -101,6 +101,24 @@ def test_recycled_slot_handed_out_twice_is_detected(batch, orc, variant):
     assert r.events["double_alloc"] > 0
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_mid_turn_pool_is_clean(batch, orc, seed):
+    # the mid-turn pool (a build with HVM_MID_POOL): busy waves hand surplus
+    # redexes to the output bag, waves that run dry take them back by
+    # exchanges with 0, and the stores of handed-over redexes land several
+    # scheduling points after their reservation.  On a net 4 times larger:
+    # on the 16K-slot one the pool's extra concurrency runs out of var slots
+    # at most seeds (a capacity effect: the larger net passes every seed).
+    words, entries, nets, itrs = batch
+    cfg = dict(CFG, node_cap=1 << 16, vars_cap=1 << 16)
+    r = orc.run_model(words, entries, seed=seed, variant=orc.MODEL_POOL, **cfg)
+    assert r.status == 0 and r.err == 0
+    assert {k: r.events[k] for k in ANOMALIES} == {k: 0 for k in ANOMALIES}
+    assert r.events["pool_gives"] > 0 and r.events["pool_takes"] > 0   # the pool is exercised
+    assert r.itrs == itrs
+    assert all(result_ok(t, e) for t, (_, e) in zip(r.results, nets))
+
+
 @pytest.mark.parametrize("seed", [4, 5])
 def test_batch_with_var_recycling_is_clean(batch, orc, seed):
     # var slots freed by links recycled as well (a build with HVM_VAR_RECYCLE)
