@@ -42,8 +42,9 @@ static_assert((HVM_QUIT_EVERY & (HVM_QUIT_EVERY - 1u)) == 0 && HVM_QUIT_EVERY <=
 // Mid-turn pool (HBM evaluator, HVM_MID_POOL): at its idle-quit looks a busy
 // wave hands the oldest redex of every lane holding three or more to the
 // output bag while fewer than HVM_POOL_LOW handed-over redexes wait there, and
-// a wave that runs dry first claims a wave's worth of output-bag slots (a
-// cursor in HvmCtl) and takes each by an exchange with 0.  Bag slots outside
+// a wave that runs dry first claims up to a wave's worth of the handed-over
+// slots (a cursor in HvmCtl, advanced by compare-and-swap to no further than
+// the bag's count) and takes each by an exchange with 0.  Bag slots outside
 // the live range are kept zero for this (every input-bag read is an exchange
 // with 0, and the runtime clears both bags with the heap), so a slot whose
 // store has not landed reads 0 and keeps its redex for the next turn, which
@@ -810,14 +811,24 @@ __device__ __forceinline__ void turn_body(const HvmTurnArgs& A, WaveLds& L, Loca
       if (POOL && !evac) {
         // before counting as idle: up to a wave of the redexes busy waves
         // handed to the output bag this turn
-        const u32 out_n = __hip_atomic_load(&A.ctl->n_out, __ATOMIC_RELAXED, AGENT);
-        const u32 took = __hip_atomic_load(&A.ctl->out_take, __ATOMIC_RELAXED, AGENT);
-        if (out_n > took) {
-          u32 t = 0;
-          if (lane == 0) t = atomicAdd(&A.ctl->out_take, (u32)HVM_WAVE);
-          t = lane_u32(t, 0);
+        // claim only what waits (a compare-and-swap of the cursor, a few
+        // tries), so that the cursor never runs past the redexes handed over
+        u32 t = 0, k = 0;
+        if (lane == 0) {
+          const u32 out_n = min(__hip_atomic_load(&A.ctl->n_out, __ATOMIC_RELAXED, AGENT), A.bag_cap);
+          u32 cur = __hip_atomic_load(&A.ctl->out_take, __ATOMIC_RELAXED, AGENT);
+          for (int tries = 0; tries < 4 && cur < out_n; ++tries) {
+            const u32 want = min((u32)HVM_WAVE, out_n - cur);
+            const u32 prev = atomicCAS(&A.ctl->out_take, cur, cur + want);
+            if (prev == cur) { t = cur; k = want; break; }
+            cur = prev;
+          }
+        }
+        t = lane_u32(t, 0);
+        k = lane_u32(k, 0);
+        if (k) {
           u64 x = 0;
-          if (t + lane < min(out_n, A.bag_cap)) x = __hip_atomic_exchange(&bag_out[t + lane], 0ull, __ATOMIC_RELAXED, AGENT);
+          if ((u32)lane < k) x = __hip_atomic_exchange(&bag_out[t + lane], 0ull, __ATOMIC_RELAXED, AGENT);
           if (x) push_redex<LOC>(L, N, ln, A, x);
           if (__ballot(x != 0)) continue;
         }

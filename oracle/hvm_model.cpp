@@ -580,15 +580,16 @@ void Model::turn(u32 w) {
       if (pool) {
         // before counting as idle: a wave's worth of handed-over redexes
         yield();
-        const u32 out_n = n_out;
+        const u32 out_n = std::min(n_out, C.bag_cap);
         if (out_n > out_take) {
+          // claim only what waits (the kernel's compare-and-swap)
           const u32 t = out_take;
-          out_take += L;
+          const u32 k = std::min(L, out_n - t);
+          out_take += k;
           yield();
           bool got = false;
-          for (u32 r = 0; r < L; ++r) {
+          for (u32 r = 0; r < k; ++r) {
             const u32 i = t + r;
-            if (i >= std::min(out_n, C.bag_cap)) continue;
             const u64 x = bag_out[i];
             bag_out[i] = 0;                    // the exchange with 0
             if (x) { push_redex(lanes[r], x, spills); got = true; O->pool_takes++; }
