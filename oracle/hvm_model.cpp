@@ -1066,12 +1066,17 @@ extern "C" int hvm_model_run(const uint32_t* book, size_t n_words, const uint32_
     // between turns (hvm_eval.hip: hvm_between_kernel, plain scheduler)
     u32 n = m.n_out;
     if (n > C.bag_cap) { m.err |= ERR_BAG; break; }
-    bool steady = n <= C.waves / 4 && 2 * n <= 3 * m.n_in;
+    const bool flat = 2 * n <= 3 * m.n_in;
+    bool steady = n <= C.waves / 4 && flat;
     m.n_in = n;
     m.in_head = std::min(n, C.waves * C.lanes);
     m.parity ^= 1u;
     if (n == 0) break;
     m.idle_quit = steady ? 0u : idle_cfg;
+    // the flat-frontier rule: the waves the spread leaves empty count as
+    // idle at once, the share applies to the n waves that got a redex
+    if (!steady && flat && n < C.waves && idle_cfg)
+      m.idle_quit = (C.waves - n) + std::max<u32>(1u, (u32)(((u64)idle_cfg * n) / C.waves));
   }
   if (getenv("HVM_MODEL_TRACE") && m.bad_slot != 0xFFFFFFFFu) {
     static const char* what[] = {"", "xchg", "store", "cas", "alloc", "release"};
