@@ -223,12 +223,19 @@ def test_autotuner_keeps_results(hip):
     assert not any("flags" in m.get("error", "") for m in rep["measurements"])
 
 
-@pytest.mark.parametrize("name", ["radix_lsd", "batch", "church_loop"])
+# autotune.py workload name -> generator of the run-c records
+TUNED = {"radix_lsd": "radix_lsd", "batch": None, "church_loop": "church_loop", "sum": "sum_tree",
+         "bitonic": "bitonic", "radix": "radix"}
+
+
+@pytest.mark.parametrize("name", sorted(TUNED))
 def test_tuned_configurations_keep_parity(hip, name):
-    """The autotuner's choice for the LSD radix sort of 2^18 keys, the 256-net
-    batch and church_loop(12, 12) (lambdavm-rust_amd/tuned_<name>.json),
-    compile-time variant included, evaluated in a child process that loads
-    that build: ITRS and normal forms equal the run-c records."""
+    """The autotuner's choice for each tuned workload (LSD radix sort of 2^18
+    keys, the 256-net batch, church_loop(12, 12), sum tree d=20, bitonic sort
+    of 2^16 keys, trie radix sort of 2^18 keys; lambdavm-rust_amd/
+    tuned_<name>.json), compile-time variant included, evaluated in a child
+    process that loads that build: ITRS and normal forms equal the run-c
+    records."""
     import subprocess
     import sys
     path = os.path.join(ROOT, "lambdavm-rust_amd", f"tuned_{name}.json")
@@ -253,5 +260,5 @@ def test_tuned_configurations_keep_parity(hip, name):
         assert res["nets_nf_sha256"] == [row["nf_sha256"] for row in g["nets"]]
     else:
         g = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
-        case = next(c for c in g["workloads"] if c["generator"] == name and c.get("full_size"))
+        case = next(c for c in g["workloads"] if c["generator"] == TUNED[name] and c.get("full_size"))
         assert res["ident"] == [case["itrs"], case["nf_sha256"]]
